@@ -1,0 +1,1253 @@
+/*
+ * dense_oracle.c -- CPU restatement of 2DENSE's explicit update (FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it; the
+ * product library (2dense_amd/csrc) never links or calls it.
+ *
+ * Every routine restates one reference routine with the same data layout
+ * and the same floating-point evaluation order (left-to-right, no FMA
+ * contraction: build with -ffp-contract=off), so its results are bitwise
+ * equal to the reference compiled with flang on x86-64.  Citations are
+ * relative to the reference repository root.
+ *
+ * Single-precision literals of the reference are widened exactly as
+ * Fortran widens them (e.g. 1.0e-6 -> 9.99999997475242708e-7).
+ *
+ * Op counting (-DORA_COUNT_OPS): +,-,*,/,sqrt count 1; abs, negation,
+ * max and compares count 0; loop-invariant parameter expressions
+ * (gamma-1, 2*gamma, dt/dx, ...) are not counted.
+ */
+#include "dense_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifdef ORA_COUNT_OPS
+static long long g_ops;
+#define OPS(n) (g_ops += (n))
+#else
+#define OPS(n) ((void)0)
+#endif
+
+#define BF 4
+#define NEQ 4
+#define NPRIM 6
+
+/* Single-precision literals of the reference, widened to double. */
+#define SP(x) ((double)(float)(x))
+
+/* weno_mod.f90:3-37 */
+static const double a300 = 1.0 / 3.0;
+static const double a301 = -7.0 / 6.0;
+static const double a302 = 11.0 / 6.0;
+static const double a310 = -1.0 / 6.0;
+static const double a311 = 5.0 / 6.0;
+static const double a312 = 1.0 / 3.0;
+static const double a320 = 1.0 / 3.0;
+static const double a321 = 5.0 / 6.0;
+static const double a322 = -1.0 / 6.0;
+static const double c30 = 0.1;
+static const double c31 = 0.6;
+static const double c32 = 0.3;
+#define WENO_SS SP(1.0e-6) /* weno_mod.f90:37, single literal */
+
+struct ora_block {
+    ora_params p;
+    int nip, njp; /* padded extents ni+8, nj+8 */
+    double* cons;
+    double* prim;
+    double* cons_temp;
+    double* bodyforce;
+    double *flux_xip, *flux_xin, *flux_etap, *flux_etan; /* (ni+8,nj,4) / (nj+8,ni,4) */
+    double *flux_xi, *flux_eta;                         /* SC indicators (ni+8,nj,2) / (nj+8,ni,2) */
+    double *h_xip, *h_xin, *h_etap, *h_etan, *h_xi, *h_eta;
+    double *theta_xip, *theta_xin, *theta_etap, *theta_etan; /* (4,ni+1,nj) / (4,nj+1,ni) */
+    double max_eig_i, max_eig_j;
+    double dt, current_time;
+    int current_step;
+};
+
+/* 1-based Fortran index helpers. */
+#define CONS(b, i, j, k) (b)->cons[((i)-1) + (size_t)((j)-1) * (b)->nip + (size_t)((k)-1) * (b)->nip * (b)->njp]
+#define PRIM(b, i, j, k) (b)->prim[((i)-1) + (size_t)((j)-1) * (b)->nip + (size_t)((k)-1) * (b)->nip * (b)->njp]
+#define CTMP(b, i, j, k) (b)->cons_temp[((i)-1) + (size_t)((j)-1) * (b)->nip + (size_t)((k)-1) * (b)->nip * (b)->njp]
+#define BFRC(b, i, j, k) (b)->bodyforce[((i)-1) + (size_t)((j)-1) * (b)->nip + (size_t)((k)-1) * (b)->nip * (b)->njp]
+/* flux_xi*(ni+8, nj, 4) */
+#define FX(a, b, i, j, k) (a)[((i)-1) + (size_t)((j)-1) * (b)->nip + (size_t)((k)-1) * (b)->nip * (b)->p.nj]
+/* flux_eta*(nj+8, ni, 4), transposed */
+#define FE(a, b, j, i, k) (a)[((j)-1) + (size_t)((i)-1) * (b)->njp + (size_t)((k)-1) * (b)->njp * (b)->p.ni]
+/* h_xi*(ni+1, nj, 4) */
+#define HX(a, b, i, j, k) (a)[((i)-1) + (size_t)((j)-1) * ((b)->p.ni + 1) + (size_t)((k)-1) * ((b)->p.ni + 1) * (b)->p.nj]
+/* h_eta*(nj+1, ni, 4) */
+#define HE(a, b, j, i, k) (a)[((j)-1) + (size_t)((i)-1) * ((b)->p.nj + 1) + (size_t)((k)-1) * ((b)->p.nj + 1) * (b)->p.ni]
+/* theta_xi*(4, ni+1, nj) */
+#define TX(a, b, m, i, j) (a)[((m)-1) + 4 * (((i)-1) + (size_t)((j)-1) * ((b)->p.ni + 1))]
+/* theta_eta*(4, nj+1, ni) */
+#define TE(a, b, m, j, i) (a)[((m)-1) + 4 * (((j)-1) + (size_t)((i)-1) * ((b)->p.nj + 1))]
+
+static int nthreads_of(const ora_block* b) { return b->p.nthreads > 0 ? b->p.nthreads : 1; }
+
+/* ------------------------------------------------------------------ */
+/* Scalar stencils: weno_mod.f90                                       */
+/* ------------------------------------------------------------------ */
+
+/* smoothness indicators, weno_mod.f90:152-159 (JS) and :175-182 (Z) */
+static inline void smooth_ind(const double* lf, double* is0, double* is1, double* is2)
+{
+    double x, y;
+    x = (lf[0] - 2.0 * lf[1]) + lf[2];
+    y = (lf[0] - 4.0 * lf[1]) + 3.0 * lf[2];
+    *is0 = 13.0 * (x * x) / 12.0 + (y * y) / 4.0;
+    x = (lf[1] - 2.0 * lf[2]) + lf[3];
+    y = lf[1] - lf[3];
+    *is1 = 13.0 * (x * x) / 12.0 + (y * y) / 4.0;
+    x = (lf[2] - 2.0 * lf[3]) + lf[4];
+    y = (3.0 * lf[2] - 4.0 * lf[3]) + lf[4];
+    *is2 = 13.0 * (x * x) / 12.0 + (y * y) / 4.0;
+    OPS(36);
+}
+
+/* weno5z_stencil_alpha, weno_mod.f90:168-188.  lf[0..4] = lf(-2..2). */
+static inline void weno5z_alpha(const double* lf, double* aa0, double* aa1, double* aa2)
+{
+    double is0, is1, is2, tau5, t;
+    smooth_ind(lf, &is0, &is1, &is2);
+    tau5 = fabs(is2 - is0);
+    t = tau5 / (WENO_SS + is0);
+    *aa0 = c30 * (1.0 + t * t);
+    t = tau5 / (WENO_SS + is1);
+    *aa1 = c31 * (1.0 + t * t);
+    t = tau5 / (WENO_SS + is2);
+    *aa2 = c32 * (1.0 + t * t);
+    OPS(16);
+}
+
+/* weno5js_stencil_alpha, weno_mod.f90:145-165 */
+static inline void weno5js_alpha(const double* lf, double* aa0, double* aa1, double* aa2)
+{
+    double is0, is1, is2, s;
+    smooth_ind(lf, &is0, &is1, &is2);
+    s = WENO_SS + is0;
+    *aa0 = c30 / (s * s);
+    s = WENO_SS + is1;
+    *aa1 = c31 / (s * s);
+    s = WENO_SS + is2;
+    *aa2 = c32 / (s * s);
+    OPS(9);
+}
+
+/* weno5z_stencil_omega_only / _theta, weno_mod.f90:67-95 */
+static inline void weno5z_omega(const double* lf, double* w0, double* w1, double* w2, double* theta)
+{
+    double aa0, aa1, aa2, s;
+    weno5z_alpha(lf, &aa0, &aa1, &aa2);
+    s = aa0 + aa1 + aa2;
+    *w0 = aa0 / s;
+    *w1 = aa1 / s;
+    *w2 = aa2 / s;
+    OPS(5);
+    if (theta) {
+        /* theta = 1.0/(1.0+(aa_sum-1.0)**2), weno_mod.f90:94 */
+        double d = s - 1.0;
+        *theta = 1.0 / (1.0 + d * d);
+        OPS(4);
+    }
+}
+
+/* candidate polynomials, weno_mod.f90:57-59 */
+static inline void candidates(const double* lf, double* q0, double* q1, double* q2)
+{
+    *q0 = a300 * lf[0] + a301 * lf[1] + a302 * lf[2];
+    *q1 = a310 * lf[1] + a311 * lf[2] + a312 * lf[3];
+    *q2 = a320 * lf[2] + a321 * lf[3] + a322 * lf[4];
+    OPS(15);
+}
+
+/* weno5z_stencil, weno_mod.f90:50-65 */
+static inline double weno5z(const double* lf)
+{
+    double q0, q1, q2, w0, w1, w2;
+    candidates(lf, &q0, &q1, &q2);
+    weno5z_omega(lf, &w0, &w1, &w2, NULL);
+    OPS(5);
+    return w0 * q0 + w1 * q1 + w2 * q2;
+}
+
+/* weno5js_stencil, weno_mod.f90:98-111 */
+static inline double weno5js(const double* lf)
+{
+    double q0, q1, q2, aa0, aa1, aa2, s, w0, w1, w2;
+    candidates(lf, &q0, &q1, &q2);
+    weno5js_alpha(lf, &aa0, &aa1, &aa2);
+    s = aa0 + aa1 + aa2;
+    w0 = aa0 / s;
+    w1 = aa1 / s;
+    w2 = aa2 / s;
+    OPS(10);
+    return w0 * q0 + w1 * q1 + w2 * q2;
+}
+
+/* scheme_upwind5_general.f90:86-90 */
+static inline double upwind5(const double* lf)
+{
+    OPS(10);
+    return (2.0 * lf[0] - 13.0 * lf[1] + 47.0 * lf[2] + 27.0 * lf[3] - 3.0 * lf[4]) / 60.0;
+}
+
+double ora_weno5z_stencil(const double lf[5]) { return weno5z(lf); }
+double ora_weno5js_stencil(const double lf[5]) { return weno5js(lf); }
+double ora_upwind5_stencil(const double lf[5]) { return upwind5(lf); }
+void ora_weno5z_omega_theta(const double lf[5], double out[4])
+{
+    weno5z_omega(lf, &out[1], &out[2], &out[3], &out[0]);
+}
+
+/* ------------------------------------------------------------------ */
+/* Eigenvectors: get_l_r_vec_xi / _eta, scheme_weno5th_SC.f90:277-328, */
+/* :607-658.  l(comp,char), r(char,comp), column-major 4x4.            */
+/* ------------------------------------------------------------------ */
+#define M4(a, r, c) (a)[((r)-1) + 4 * ((c)-1)]
+
+static void lr_vec_xi(double u0, double v0, double h0, double alpha0, double gamma, double* l, double* r)
+{
+    double uu0 = 0.5 * (u0 * u0 + v0 * v0);
+    double hh0 = alpha0 * alpha0 / (gamma - 1.0);
+    OPS(5);
+    M4(r, 1, 1) = 1.0; M4(r, 2, 1) = 1.0; M4(r, 3, 1) = 0.0; M4(r, 4, 1) = 1.0;
+    M4(r, 1, 2) = u0 - alpha0; M4(r, 2, 2) = u0; M4(r, 3, 2) = 0.0; M4(r, 4, 2) = u0 + alpha0;
+    M4(r, 1, 3) = v0; M4(r, 2, 3) = v0; M4(r, 3, 3) = 1.0; M4(r, 4, 3) = v0;
+    M4(r, 1, 4) = h0 - alpha0 * u0; M4(r, 2, 4) = uu0; M4(r, 3, 4) = v0; M4(r, 4, 4) = h0 + alpha0 * u0;
+    OPS(5); /* u0-a0, u0+a0, a0*u0 (shared), h0-, h0+ */
+
+    M4(l, 1, 1) = 0.5 * uu0 / hh0 + 0.5 * u0 / alpha0;
+    M4(l, 2, 1) = -(0.5 * u0 / hh0) - 0.5 / alpha0;
+    M4(l, 3, 1) = -(0.5 * v0 / hh0);
+    M4(l, 4, 1) = 0.5 / hh0;
+
+    M4(l, 1, 2) = -(uu0 / hh0) + 1.0;
+    M4(l, 2, 2) = u0 / hh0;
+    M4(l, 3, 2) = v0 / hh0;
+    M4(l, 4, 2) = -(1.0 / hh0);
+
+    M4(l, 1, 3) = -v0; M4(l, 2, 3) = 0.0; M4(l, 3, 3) = 1.0; M4(l, 4, 3) = 0.0;
+
+    M4(l, 1, 4) = 0.5 * uu0 / hh0 - 0.5 * u0 / alpha0;
+    M4(l, 2, 4) = -(0.5 * u0 / hh0) + 0.5 / alpha0;
+    M4(l, 3, 4) = -(0.5 * v0 / hh0);
+    M4(l, 4, 4) = 0.5 / hh0;
+    /* distinct values: 0.5uu0/hh0 (2), 0.5u0/a0 (2), 0.5u0/hh0 (2), 0.5/a0 (1),
+       0.5v0/hh0 (2), 0.5/hh0 (1), uu0/hh0 (1), u0/hh0, v0/hh0, 1/hh0 (3); sums 5 */
+    OPS(22);
+}
+
+static void lr_vec_eta(double u0, double v0, double h0, double alpha0, double gamma, double* l, double* r)
+{
+    double uu0 = 0.5 * (u0 * u0 + v0 * v0);
+    double hh0 = alpha0 * alpha0 / (gamma - 1.0);
+    OPS(5);
+    M4(r, 1, 1) = 1.0; M4(r, 2, 1) = 1.0; M4(r, 3, 1) = 0.0; M4(r, 4, 1) = 1.0;
+    M4(r, 1, 2) = u0; M4(r, 2, 2) = u0; M4(r, 3, 2) = 1.0; M4(r, 4, 2) = u0;
+    M4(r, 1, 3) = v0 - alpha0; M4(r, 2, 3) = v0; M4(r, 3, 3) = 0.0; M4(r, 4, 3) = v0 + alpha0;
+    M4(r, 1, 4) = h0 - alpha0 * v0; M4(r, 2, 4) = uu0; M4(r, 3, 4) = u0; M4(r, 4, 4) = h0 + alpha0 * v0;
+    OPS(5);
+
+    M4(l, 1, 1) = 0.5 * uu0 / hh0 + 0.5 * v0 / alpha0;
+    M4(l, 2, 1) = -(0.5 * u0 / hh0);
+    M4(l, 3, 1) = -(0.5 * v0 / hh0) - 0.5 / alpha0;
+    M4(l, 4, 1) = 0.5 / hh0;
+
+    M4(l, 1, 2) = -(uu0 / hh0) + 1.0;
+    M4(l, 2, 2) = u0 / hh0;
+    M4(l, 3, 2) = v0 / hh0;
+    M4(l, 4, 2) = -(1.0 / hh0);
+
+    M4(l, 1, 3) = -u0; M4(l, 2, 3) = 1.0; M4(l, 3, 3) = 0.0; M4(l, 4, 3) = 0.0;
+
+    M4(l, 1, 4) = 0.5 * uu0 / hh0 - 0.5 * v0 / alpha0;
+    M4(l, 2, 4) = -(0.5 * u0 / hh0);
+    M4(l, 3, 4) = -(0.5 * v0 / hh0) + 0.5 / alpha0;
+    M4(l, 4, 4) = 0.5 / hh0;
+    OPS(22);
+}
+
+void ora_lr_vec_xi(double u0, double v0, double h0, double a0, double gamma, double l[16], double r[16])
+{
+    lr_vec_xi(u0, v0, h0, a0, gamma, l, r);
+}
+void ora_lr_vec_eta(double u0, double v0, double h0, double a0, double gamma, double l[16], double r[16])
+{
+    lr_vec_eta(u0, v0, h0, a0, gamma, l, r);
+}
+
+/* Roe average of two interface cells, scheme_weno5th_CH.f90:44-62.
+ * Inputs: (rho,u,v,p) left and right. */
+static inline void roe_average(double gamma, double rhol, double ul, double vl, double pl,
+                               double rhor, double ur, double vr, double pr,
+                               double* u0, double* v0, double* h0, double* alpha0)
+{
+    double hl = pl / (gamma - 1) / rhol + 0.5 * (ul * ul + vl * vl) + pl / rhol;
+    double hr = pr / (gamma - 1) / rhor + 0.5 * (ur * ur + vr * vr) + pr / rhor;
+    double sr = sqrt(rhor), sl = sqrt(rhol);
+    *u0 = (sr * ur + sl * ul) / (sr + sl);
+    *v0 = (sr * vr + sl * vl) / (sr + sl);
+    *h0 = (sr * hr + sl * hl) / (sr + sl);
+    *alpha0 = sqrt((gamma - 1.0) * (*h0 - 0.5 * (*u0 * *u0 + *v0 * *v0)));
+    /* hl,hr: 9 each; sqrt x2; u0,v0,h0: 4 each + shared denominator 1; alpha0: 7 */
+    OPS(18 + 2 + 13 + 7);
+}
+
+/* ------------------------------------------------------------------ */
+/* get_primary, get_primary.f90:1-78                                   */
+/* ------------------------------------------------------------------ */
+static void get_primary(ora_block* b)
+{
+    const int ni = b->p.ni, nj = b->p.nj;
+    const double gamma = b->p.gamma;
+    double mi = 0.0, mj = 0.0;
+    int j;
+#pragma omp parallel for num_threads(nthreads_of(b)) reduction(max : mi, mj) schedule(static)
+    for (j = BF + 1; j <= nj + BF; j++) {
+        for (int i = BF + 1; i <= ni + BF; i++) {
+            double rho = CONS(b, i, j, 1);
+            double u = CONS(b, i, j, 2) / rho;
+            double v = CONS(b, i, j, 3) / rho;
+            double p = (gamma - 1.0) * (CONS(b, i, j, 4) - 0.5 * rho * (u * u + v * v));
+            if (p < 0.0) p = SP(1.0e-6); /* get_primary.f90:57-59 */
+            double alpha = sqrt(gamma * p / rho);
+            PRIM(b, i, j, 1) = rho;
+            PRIM(b, i, j, 2) = u;
+            PRIM(b, i, j, 3) = v;
+            PRIM(b, i, j, 4) = p;
+            PRIM(b, i, j, 5) = alpha;
+            PRIM(b, i, j, 6) = log(p) - log(rho);
+            double ei = fabs(u) + alpha, ej = fabs(v) + alpha;
+            if (ei > mi) mi = ei;
+            if (ej > mj) mj = ej;
+        }
+    }
+    OPS((long long)ni * nj * 14);
+    b->max_eig_i = mi;
+    b->max_eig_j = mj;
+}
+
+/* ------------------------------------------------------------------ */
+/* Boundary conditions.  Each side kind restates one reference routine: */
+/*  periodic      boundary_ivc_general.f90:13-35 (y), boundary_rt_general.f90:129-151 (x) */
+/*  extrapolation boundary_ivc_general.f90:61-102, boundary_doublemach_general.f90:120-139 */
+/*  reflective    boundary_shockvortex_general.f90:15-79                 */
+/*  Dirichlet     boundary_rt_general.f90:14-57, boundary_doublemach_general.f90:98-118 */
+/*  DMR wall/top  boundary_doublemach_general.f90:15-96                  */
+/* ------------------------------------------------------------------ */
+static void set_dirichlet(ora_block* b, int i, int j, double rho, double u, double v, double p)
+{
+    const double gamma = b->p.gamma;
+    CONS(b, i, j, 1) = rho;
+    CONS(b, i, j, 2) = rho * u;
+    CONS(b, i, j, 3) = rho * v;
+    CONS(b, i, j, 4) = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v);
+    PRIM(b, i, j, 1) = rho;
+    PRIM(b, i, j, 2) = u;
+    PRIM(b, i, j, 3) = v;
+    PRIM(b, i, j, 4) = p;
+    PRIM(b, i, j, 5) = sqrt(gamma * p / rho);
+}
+
+/* mirror cell (i,js) into ghost (i,jg) (y-normal) */
+static void reflect_y(ora_block* b, int i, int jg, int js)
+{
+    CONS(b, i, jg, 1) = CONS(b, i, js, 1);
+    CONS(b, i, jg, 2) = CONS(b, i, js, 2);
+    CONS(b, i, jg, 3) = -CONS(b, i, js, 3);
+    CONS(b, i, jg, 4) = CONS(b, i, js, 4);
+    PRIM(b, i, jg, 1) = PRIM(b, i, js, 1);
+    PRIM(b, i, jg, 2) = PRIM(b, i, js, 2);
+    PRIM(b, i, jg, 3) = -PRIM(b, i, js, 3);
+    PRIM(b, i, jg, 4) = PRIM(b, i, js, 4);
+    PRIM(b, i, jg, 5) = PRIM(b, i, js, 5);
+}
+
+/* x-normal mirror (reflective left/right; the reference has no such side,
+ * the rotation of reflect_y is used for symmetric configurations) */
+static void reflect_x(ora_block* b, int ig, int is, int j)
+{
+    CONS(b, ig, j, 1) = CONS(b, is, j, 1);
+    CONS(b, ig, j, 2) = -CONS(b, is, j, 2);
+    CONS(b, ig, j, 3) = CONS(b, is, j, 3);
+    CONS(b, ig, j, 4) = CONS(b, is, j, 4);
+    PRIM(b, ig, j, 1) = PRIM(b, is, j, 1);
+    PRIM(b, ig, j, 2) = -PRIM(b, is, j, 2);
+    PRIM(b, ig, j, 3) = PRIM(b, is, j, 3);
+    PRIM(b, ig, j, 4) = PRIM(b, is, j, 4);
+    PRIM(b, ig, j, 5) = PRIM(b, is, j, 5);
+}
+
+static void boundary(ora_block* b)
+{
+    const int ni = b->p.ni, nj = b->p.nj;
+    const double dx = b->p.dx;
+    int i, j, k, q;
+
+    /* left (x-) */
+    switch (b->p.bc[0]) {
+    case ORA_BC_PERIODIC:
+        for (k = 1; k <= NEQ; k++)
+            for (j = BF + 1; j <= nj + BF; j++)
+                for (i = 1; i <= BF; i++) CONS(b, i, j, k) = CONS(b, ni + i, j, k);
+        for (k = 1; k <= NPRIM; k++)
+            for (j = BF + 1; j <= nj + BF; j++)
+                for (i = 1; i <= BF; i++) PRIM(b, i, j, k) = PRIM(b, ni + i, j, k);
+        break;
+    case ORA_BC_EXTRAP:
+        for (k = 1; k <= NEQ; k++)
+            for (j = BF + 1; j <= nj + BF; j++)
+                for (i = BF; i >= 1; i--)
+                    CONS(b, i, j, k) = CONS(b, i + 1, j, k) * 4.0 / 3.0 - CONS(b, i + 2, j, k) / 3.0;
+        for (k = 1; k <= NPRIM; k++)
+            for (j = BF + 1; j <= nj + BF; j++)
+                for (i = BF; i >= 1; i--)
+                    PRIM(b, i, j, k) = PRIM(b, i + 1, j, k) * 4.0 / 3.0 - PRIM(b, i + 2, j, k) / 3.0;
+        break;
+    case ORA_BC_REFLECT:
+        for (j = BF + 1; j <= nj + BF; j++)
+            for (i = 1; i <= BF; i++) reflect_x(b, i, 2 * BF - i + 1, j);
+        break;
+    case ORA_BC_DIRICHLET:
+        for (j = BF + 1; j <= nj + BF; j++)
+            for (i = 1; i <= BF; i++)
+                set_dirichlet(b, i, j, b->p.bc_state[0][0], b->p.bc_state[0][1], b->p.bc_state[0][2],
+                              b->p.bc_state[0][3]);
+        break;
+    }
+    /* right (x+) */
+    switch (b->p.bc[1]) {
+    case ORA_BC_PERIODIC:
+        for (k = 1; k <= NEQ; k++)
+            for (j = BF + 1; j <= nj + BF; j++)
+                for (i = 1; i <= BF; i++) CONS(b, ni + BF + i, j, k) = CONS(b, BF + i, j, k);
+        for (k = 1; k <= NPRIM; k++)
+            for (j = BF + 1; j <= nj + BF; j++)
+                for (i = 1; i <= BF; i++) PRIM(b, ni + BF + i, j, k) = PRIM(b, BF + i, j, k);
+        break;
+    case ORA_BC_EXTRAP:
+        for (k = 1; k <= NEQ; k++)
+            for (j = BF + 1; j <= nj + BF; j++)
+                for (i = ni + BF + 1; i <= ni + 2 * BF; i++)
+                    CONS(b, i, j, k) = CONS(b, i - 1, j, k) * 4.0 / 3.0 - CONS(b, i - 2, j, k) / 3.0;
+        for (k = 1; k <= NPRIM; k++)
+            for (j = BF + 1; j <= nj + BF; j++)
+                for (i = ni + BF + 1; i <= ni + 2 * BF; i++)
+                    PRIM(b, i, j, k) = PRIM(b, i - 1, j, k) * 4.0 / 3.0 - PRIM(b, i - 2, j, k) / 3.0;
+        break;
+    case ORA_BC_REFLECT:
+        for (j = BF + 1; j <= nj + BF; j++)
+            for (i = ni + BF + 1; i <= ni + 2 * BF; i++) reflect_x(b, i, 2 * ni + 2 * BF - i + 1, j);
+        break;
+    case ORA_BC_DIRICHLET:
+        for (j = BF + 1; j <= nj + BF; j++)
+            for (i = ni + BF + 1; i <= ni + 2 * BF; i++)
+                set_dirichlet(b, i, j, b->p.bc_state[1][0], b->p.bc_state[1][1], b->p.bc_state[1][2],
+                              b->p.bc_state[1][3]);
+        break;
+    }
+    /* bottom (y-) */
+    switch (b->p.bc[2]) {
+    case ORA_BC_PERIODIC:
+        for (k = 1; k <= NEQ; k++)
+            for (j = 1; j <= BF; j++)
+                for (i = BF + 1; i <= ni + BF; i++) CONS(b, i, j, k) = CONS(b, i, nj + j, k);
+        for (k = 1; k <= NPRIM; k++)
+            for (j = 1; j <= BF; j++)
+                for (i = BF + 1; i <= ni + BF; i++) PRIM(b, i, j, k) = PRIM(b, i, nj + j, k);
+        break;
+    case ORA_BC_EXTRAP:
+        for (k = 1; k <= NEQ; k++)
+            for (i = BF + 1; i <= ni + BF; i++)
+                for (j = BF; j >= 1; j--)
+                    CONS(b, i, j, k) = CONS(b, i, j + 1, k) * 4.0 / 3.0 - CONS(b, i, j + 2, k) / 3.0;
+        for (k = 1; k <= NPRIM; k++)
+            for (i = BF + 1; i <= ni + BF; i++)
+                for (j = BF; j >= 1; j--)
+                    PRIM(b, i, j, k) = PRIM(b, i, j + 1, k) * 4.0 / 3.0 - PRIM(b, i, j + 2, k) / 3.0;
+        break;
+    case ORA_BC_REFLECT:
+        for (j = 1; j <= BF; j++)
+            for (i = BF + 1; i <= ni + BF; i++) reflect_y(b, i, j, 2 * BF - j + 1);
+        break;
+    case ORA_BC_DIRICHLET:
+        for (j = 1; j <= BF; j++)
+            for (i = BF + 1; i <= ni + BF; i++)
+                set_dirichlet(b, i, j, b->p.bc_state[2][0], b->p.bc_state[2][1], b->p.bc_state[2][2],
+                              b->p.bc_state[2][3]);
+        break;
+    case ORA_BC_DMR_WALL:
+        for (j = 1; j <= BF; j++)
+            for (i = BF + 1; i <= ni + BF; i++) {
+                double x = ((double)(i - BF) - 0.5) * dx; /* x(index_i-bfsize_i,1) */
+                if (x >= 1.0 / 6.0)
+                    reflect_y(b, i, j, 2 * BF - j + 1);
+                else
+                    set_dirichlet(b, i, j, b->p.bc_state[2][0], b->p.bc_state[2][1], b->p.bc_state[2][2],
+                                  b->p.bc_state[2][3]);
+            }
+        break;
+    }
+    /* top (y+) */
+    switch (b->p.bc[3]) {
+    case ORA_BC_PERIODIC:
+        for (k = 1; k <= NEQ; k++)
+            for (j = 1; j <= BF; j++)
+                for (i = BF + 1; i <= ni + BF; i++) CONS(b, i, nj + BF + j, k) = CONS(b, i, BF + j, k);
+        for (k = 1; k <= NPRIM; k++)
+            for (j = 1; j <= BF; j++)
+                for (i = BF + 1; i <= ni + BF; i++) PRIM(b, i, nj + BF + j, k) = PRIM(b, i, BF + j, k);
+        break;
+    case ORA_BC_EXTRAP:
+        for (k = 1; k <= NEQ; k++)
+            for (i = BF + 1; i <= ni + BF; i++)
+                for (j = nj + BF + 1; j <= nj + 2 * BF; j++)
+                    CONS(b, i, j, k) = CONS(b, i, j - 1, k) * 4.0 / 3.0 - CONS(b, i, j - 2, k) / 3.0;
+        for (k = 1; k <= NPRIM; k++)
+            for (i = BF + 1; i <= ni + BF; i++)
+                for (j = nj + BF + 1; j <= nj + 2 * BF; j++)
+                    PRIM(b, i, j, k) = PRIM(b, i, j - 1, k) * 4.0 / 3.0 - PRIM(b, i, j - 2, k) / 3.0;
+        break;
+    case ORA_BC_REFLECT:
+        for (j = nj + BF + 1; j <= nj + 2 * BF; j++)
+            for (i = BF + 1; i <= ni + BF; i++) reflect_y(b, i, j, 2 * nj + 2 * BF - j + 1);
+        break;
+    case ORA_BC_DIRICHLET:
+        for (j = nj + BF + 1; j <= nj + 2 * BF; j++)
+            for (i = BF + 1; i <= ni + BF; i++)
+                set_dirichlet(b, i, j, b->p.bc_state[3][0], b->p.bc_state[3][1], b->p.bc_state[3][2],
+                              b->p.bc_state[3][3]);
+        break;
+    case ORA_BC_DMR_TOP: {
+        /* boundary_doublemach_general.f90:65-96: shock position at t = current_time */
+        const double xs = 1.0 / 6.0 + (1.0 + 20.0 * b->current_time) / sqrt(3.0);
+        for (j = nj + BF + 1; j <= nj + 2 * BF; j++)
+            for (i = BF + 1; i <= ni + BF; i++) {
+                double x = ((double)(i - BF) - 0.5) * dx; /* x(index_i-bfsize_i,nj) */
+                if (x < xs)
+                    set_dirichlet(b, i, j, b->p.bc_state[3][0], b->p.bc_state[3][1], b->p.bc_state[3][2],
+                                  b->p.bc_state[3][3]);
+                else
+                    set_dirichlet(b, i, j, 1.4, 0.0, 0.0, 1.0);
+            }
+        break;
+    }
+    }
+    (void)q;
+
+    /* boundary_rt_general.f90:154-155: body force refreshed from cons */
+    if (b->p.if_bodyforce) {
+        size_t n = (size_t)b->nip * b->njp;
+        for (size_t c = 0; c < n; c++) {
+            b->bodyforce[2 * n + c] = b->cons[c];
+            b->bodyforce[3 * n + c] = b->cons[2 * n + c];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Flux splitting: flux_splitting_{LLF,GLF,SW}.f90, CH/SC block drivers */
+/* ------------------------------------------------------------------ */
+
+/* LF-type split of one cell: alpha = |u|+c (LLF) or global (GLF).
+ * flux_splitting_LLF.f90:37-47 / flux_splitting_GLF.f90:31-43.
+ * nrm: 2 for x (u normal, p in comp 2), 3 for y (v normal, p in comp 3). */
+static inline void lf_split(double un, double p, double a, const double* c, int nrm, double* fp, double* fn)
+{
+    double up = un + a, um = un - a;
+    fp[0] = 0.5 * (up * c[0]);
+    fn[0] = 0.5 * (um * c[0]);
+    if (nrm == 2) {
+        fp[1] = 0.5 * (up * c[1] + p);
+        fp[2] = 0.5 * (up * c[2]);
+        fn[1] = 0.5 * (um * c[1] + p);
+        fn[2] = 0.5 * (um * c[2]);
+    } else {
+        fp[1] = 0.5 * (up * c[1]);
+        fp[2] = 0.5 * (up * c[2] + p);
+        fn[1] = 0.5 * (um * c[1]);
+        fn[2] = 0.5 * (um * c[2] + p);
+    }
+    fp[3] = 0.5 * (up * c[3] + un * p);
+    fn[3] = 0.5 * (um * c[3] + un * p);
+    OPS(24);
+}
+
+/* Steger-Warming split of one cell, flux_splitting_SW.f90:36-92 (x) and
+ * :103-161 (y).  un normal, ut tangential velocity. */
+static inline void sw_split(double gamma, double rho, double u, double v, double alpha, int nrm, double* fp,
+                            double* fn)
+{
+    const double epsl = 0.125;
+    double un = (nrm == 2) ? u : v;
+    double uu2 = u * u + v * v;
+    double lam[3] = {un, un - alpha, un + alpha};
+    double lp[3], wii, s;
+    OPS(5);
+    for (int sgn = 0; sgn < 2; sgn++) {
+        double* f = sgn == 0 ? fp : fn;
+        for (int q = 0; q < 3; q++) {
+            double sq = sqrt(lam[q] * lam[q] + epsl * epsl);
+            lp[q] = sgn == 0 ? (lam[q] + sq) / 2.0 : (lam[q] - sq) / 2.0;
+        }
+        wii = (3.0 - gamma) * (lp[1] + lp[2]) * (alpha * alpha) / (2.0 * (gamma - 1.0));
+        f[0] = (lp[0] * 2.0 * (gamma - 1.0) + lp[1] + lp[2]) * rho / (2.0 * gamma);
+        if (nrm == 2) {
+            f[1] = (lp[0] * 2.0 * (gamma - 1.0) * u + lp[1] * (u - alpha) + lp[2] * (u + alpha)) * rho /
+                   (2.0 * gamma);
+            f[2] = (lp[0] * 2.0 * (gamma - 1.0) * v + lp[1] * v + lp[2] * v) * rho / (2.0 * gamma);
+            f[3] = (lp[0] * (gamma - 1.0) * uu2 + lp[1] * 0.5 * ((u - alpha) * (u - alpha) + v * v) +
+                    lp[2] * 0.5 * ((u + alpha) * (u + alpha) + v * v) + wii) *
+                   rho / (2.0 * gamma);
+        } else {
+            f[1] = (lp[0] * 2.0 * (gamma - 1.0) * u + lp[1] * u + lp[2] * u) * rho / (2.0 * gamma);
+            f[2] = (lp[0] * 2.0 * (gamma - 1.0) * v + lp[1] * (v - alpha) + lp[2] * (v + alpha)) * rho /
+                   (2.0 * gamma);
+            f[3] = (lp[0] * (gamma - 1.0) * uu2 + lp[1] * 0.5 * ((v - alpha) * (v - alpha) + u * u) +
+                    lp[2] * 0.5 * ((v + alpha) * (v + alpha) + u * u) + wii) *
+                   rho / (2.0 * gamma);
+        }
+        /* per sign: lp 3x(mul,add,sqrt,add,div)=15; wii 4; f0 6; f1 10; f2 8; f3 18 */
+        OPS(61);
+    }
+    (void)s;
+}
+
+/* Fill flux_xi+/- over i in [1,ni+8], j in [1,nj] and flux_eta+/- over
+ * j in [1,nj+8], i in [1,ni] (y stored transposed). */
+static void split_fluxes(ora_block* b, int kind /* 0 LLF, 1 GLF, 2 SW */, int with_indicator)
+{
+    const int ni = b->p.ni, nj = b->p.nj;
+    const double gamma = b->p.gamma;
+    int j, i;
+#pragma omp parallel for num_threads(nthreads_of(b)) schedule(static)
+    for (j = 1; j <= nj; j++) {
+        for (int ii = 1; ii <= ni + 2 * BF; ii++) {
+            double c[4], fp[4], fn[4];
+            for (int k = 0; k < 4; k++) c[k] = CONS(b, ii, j + BF, k + 1);
+            double rho = PRIM(b, ii, j + BF, 1), u = PRIM(b, ii, j + BF, 2), v = PRIM(b, ii, j + BF, 3);
+            double p = PRIM(b, ii, j + BF, 4), al = PRIM(b, ii, j + BF, 5);
+            if (kind == 2) {
+                sw_split(gamma, rho, u, v, al, 2, fp, fn);
+            } else {
+                double a = kind == 0 ? fabs(u) + al : b->max_eig_i;
+                if (kind == 0) OPS(1);
+                lf_split(u, p, a, c, 2, fp, fn);
+            }
+            for (int k = 0; k < 4; k++) {
+                FX(b->flux_xip, b, ii, j, k + 1) = fp[k];
+                FX(b->flux_xin, b, ii, j, k + 1) = fn[k];
+            }
+            if (with_indicator) { /* flux_splitting_SC.f90:58-59 */
+                FX(b->flux_xi, b, ii, j, 1) = rho + fp[1];
+                FX(b->flux_xi, b, ii, j, 2) = rho + fn[1];
+                OPS(2);
+            }
+        }
+    }
+#pragma omp parallel for num_threads(nthreads_of(b)) schedule(static)
+    for (i = 1; i <= ni; i++) {
+        for (int jj = 1; jj <= nj + 2 * BF; jj++) {
+            double c[4], fp[4], fn[4];
+            for (int k = 0; k < 4; k++) c[k] = CONS(b, i + BF, jj, k + 1);
+            double rho = PRIM(b, i + BF, jj, 1), u = PRIM(b, i + BF, jj, 2), v = PRIM(b, i + BF, jj, 3);
+            double p = PRIM(b, i + BF, jj, 4), al = PRIM(b, i + BF, jj, 5);
+            if (kind == 2) {
+                sw_split(gamma, rho, u, v, al, 3, fp, fn);
+            } else {
+                double a = kind == 0 ? fabs(v) + al : b->max_eig_j;
+                if (kind == 0) OPS(1);
+                lf_split(v, p, a, c, 3, fp, fn);
+            }
+            for (int k = 0; k < 4; k++) {
+                FE(b->flux_etap, b, jj, i, k + 1) = fp[k];
+                FE(b->flux_etan, b, jj, i, k + 1) = fn[k];
+            }
+            if (with_indicator) { /* flux_splitting_SC.f90:89-90 */
+                FE(b->flux_eta, b, jj, i, 1) = rho + fp[2];
+                FE(b->flux_eta, b, jj, i, 2) = rho + fn[2];
+                OPS(2);
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Component-wise reconstruction: scheme_{upwind5,weno5js,weno5z}_general.f90 */
+/* ------------------------------------------------------------------ */
+static inline double recon_scalar(int recon, const double* lf)
+{
+    switch (recon) {
+    case 251: return upwind5(lf);
+    case 253: return weno5js(lf);
+    default: return weno5z(lf);
+    }
+}
+
+static void recon_componentwise(ora_block* b)
+{
+    const int ni = b->p.ni, nj = b->p.nj, recon = b->p.recon;
+    int j, i;
+    /* scheme_weno5z_xip / _xin (scheme_weno5z_general.f90:67-117) */
+#pragma omp parallel for num_threads(nthreads_of(b)) schedule(static)
+    for (j = 1; j <= nj; j++)
+        for (int ii = 1; ii <= ni + 1; ii++)
+            for (int k = 1; k <= NEQ; k++) {
+                double lf[5];
+                for (int m = -2; m <= 2; m++) lf[m + 2] = FX(b->flux_xip, b, ii + m + BF - 1, j, k);
+                double hp = recon_scalar(recon, lf);
+                for (int m = -2; m <= 2; m++) lf[m + 2] = FX(b->flux_xin, b, ii - m + BF, j, k);
+                double hn = recon_scalar(recon, lf);
+                HX(b->h_xip, b, ii, j, k) = hp;
+                HX(b->h_xin, b, ii, j, k) = hn;
+                HX(b->h_xi, b, ii, j, k) = hp + hn; /* sum_field, field_op.f90:10-19 */
+                OPS(1);
+            }
+    /* scheme_weno5z_etap / _etan (:120-166) */
+#pragma omp parallel for num_threads(nthreads_of(b)) schedule(static)
+    for (i = 1; i <= ni; i++)
+        for (int jj = 1; jj <= nj + 1; jj++)
+            for (int k = 1; k <= NEQ; k++) {
+                double lf[5];
+                for (int m = -2; m <= 2; m++) lf[m + 2] = FE(b->flux_etap, b, jj + m + BF - 1, i, k);
+                double hp = recon_scalar(recon, lf);
+                for (int m = -2; m <= 2; m++) lf[m + 2] = FE(b->flux_etan, b, jj - m + BF, i, k);
+                double hn = recon_scalar(recon, lf);
+                HE(b->h_etap, b, jj, i, k) = hp;
+                HE(b->h_etan, b, jj, i, k) = hn;
+                HE(b->h_eta, b, jj, i, k) = hp + hn;
+                OPS(1);
+            }
+}
+
+/* ------------------------------------------------------------------ */
+/* Characteristic WENO-Z (scheme_weno5th_CH.f90) and AdaWENO            */
+/* (scheme_weno5th_SC.f90).  One face: Roe state from cells (L,R), the  */
+/* characteristic projection of the 5-point stencil of each sign.       */
+/* ------------------------------------------------------------------ */
+
+/* Characteristic reconstruction of one sign at one face.
+ * f[m][c]: flux component c at stencil point m (m=0..4 is lf(-2..2)).
+ * Projection scheme_weno5th_CH.f90:66-73, back-projection :76-81. */
+static inline void char_face(const double* l, const double* r, double f[5][4], double* h)
+{
+    double cv[4];
+    for (int k = 1; k <= 4; k++) {
+        double lf[5];
+        for (int m = 0; m < 5; m++)
+            lf[m] = M4(l, 1, k) * f[m][0] + M4(l, 2, k) * f[m][1] + M4(l, 3, k) * f[m][2] + M4(l, 4, k) * f[m][3];
+        cv[k - 1] = weno5z(lf);
+    }
+    /* projection: chars 1,2,4 have 4 non-trivial terms (7 ops), char 3 two (2 ops) per point */
+    OPS(5 * (3 * 7 + 2));
+    for (int c = 1; c <= 4; c++)
+        h[c - 1] = M4(r, 1, c) * cv[0] + M4(r, 2, c) * cv[1] + M4(r, 3, c) * cv[2] + M4(r, 4, c) * cv[3];
+    /* back-projection, non-trivial terms: rho 2, mom_n 5, mom_t 6, energy 7 */
+    OPS(20);
+}
+
+static void recon_characteristic(ora_block* b, int adaptive)
+{
+    const int ni = b->p.ni, nj = b->p.nj;
+    const double gamma = b->p.gamma, delta = b->p.delta;
+    int j, i;
+
+    if (adaptive) {
+        /* get_theta_xi_p / _xi_n, scheme_weno5th_SC.f90:1-64 */
+#pragma omp parallel for num_threads(nthreads_of(b)) schedule(static)
+        for (j = 1; j <= nj; j++)
+            for (int ii = 1; ii <= ni + 1; ii++) {
+                double lf[5];
+                for (int m = -2; m <= 2; m++) lf[m + 2] = FX(b->flux_xi, b, ii + m + BF - 1, j, 1);
+                weno5z_omega(lf, &TX(b->theta_xip, b, 2, ii, j), &TX(b->theta_xip, b, 3, ii, j),
+                             &TX(b->theta_xip, b, 4, ii, j), &TX(b->theta_xip, b, 1, ii, j));
+                for (int m = -2; m <= 2; m++) lf[m + 2] = FX(b->flux_xi, b, ii - m + BF, j, 2);
+                weno5z_omega(lf, &TX(b->theta_xin, b, 2, ii, j), &TX(b->theta_xin, b, 3, ii, j),
+                             &TX(b->theta_xin, b, 4, ii, j), &TX(b->theta_xin, b, 1, ii, j));
+            }
+    }
+
+    /* x faces: weno_character_5th_xi_p/_n, scheme_weno5th_CH.f90:1-171;
+       weno_semi_chart_5th_xi_p/_n, scheme_weno5th_SC.f90:66-275 */
+#pragma omp parallel for num_threads(nthreads_of(b)) schedule(static)
+    for (j = 1; j <= nj; j++)
+        for (int ii = 1; ii <= ni + 1; ii++) {
+            double hp[4], hn[4];
+            int cp = 1, cn = 1; /* characteristic branch taken for + / - */
+            if (adaptive) {
+                cp = !(TX(b->theta_xip, b, 1, ii, j) > delta);
+                cn = !(TX(b->theta_xin, b, 1, ii, j) > delta);
+            }
+            double l[16], r[16];
+            if (cp || cn) {
+                int il = ii + BF - 1, ir = ii + BF, jc = j + BF;
+                double u0, v0, h0, a0;
+                roe_average(gamma, PRIM(b, il, jc, 1), PRIM(b, il, jc, 2), PRIM(b, il, jc, 3), PRIM(b, il, jc, 4),
+                            PRIM(b, ir, jc, 1), PRIM(b, ir, jc, 2), PRIM(b, ir, jc, 3), PRIM(b, ir, jc, 4),
+                            &u0, &v0, &h0, &a0);
+                lr_vec_xi(u0, v0, h0, a0, gamma, l, r);
+            }
+            double f[5][4];
+            /* + sign: lf(m) = flux_xip(ii+m+bf-1) */
+            if (cp) {
+                for (int m = -2; m <= 2; m++)
+                    for (int c = 0; c < 4; c++) f[m + 2][c] = FX(b->flux_xip, b, ii + m + BF - 1, j, c + 1);
+                char_face(l, r, f, hp);
+            } else {
+                double w0 = TX(b->theta_xip, b, 2, ii, j), w1 = TX(b->theta_xip, b, 3, ii, j),
+                       w2 = TX(b->theta_xip, b, 4, ii, j);
+                for (int c = 1; c <= 4; c++) {
+                    double lf[5], q0, q1, q2;
+                    for (int m = -2; m <= 2; m++) lf[m + 2] = FX(b->flux_xip, b, ii + m + BF - 1, j, c);
+                    candidates(lf, &q0, &q1, &q2);
+                    hp[c - 1] = w0 * q0 + w1 * q1 + w2 * q2;
+                    OPS(5);
+                }
+            }
+            /* - sign: lf(m) = flux_xin(ii-m+bf) */
+            if (cn) {
+                for (int m = -2; m <= 2; m++)
+                    for (int c = 0; c < 4; c++) f[m + 2][c] = FX(b->flux_xin, b, ii - m + BF, j, c + 1);
+                char_face(l, r, f, hn);
+            } else {
+                double w0 = TX(b->theta_xin, b, 2, ii, j), w1 = TX(b->theta_xin, b, 3, ii, j),
+                       w2 = TX(b->theta_xin, b, 4, ii, j);
+                for (int c = 1; c <= 4; c++) {
+                    double lf[5], q0, q1, q2;
+                    for (int m = -2; m <= 2; m++) lf[m + 2] = FX(b->flux_xin, b, ii - m + BF, j, c);
+                    candidates(lf, &q0, &q1, &q2);
+                    hn[c - 1] = w0 * q0 + w1 * q1 + w2 * q2;
+                    OPS(5);
+                }
+            }
+            for (int c = 1; c <= 4; c++) {
+                HX(b->h_xip, b, ii, j, c) = hp[c - 1];
+                HX(b->h_xin, b, ii, j, c) = hn[c - 1];
+                HX(b->h_xi, b, ii, j, c) = hp[c - 1] + hn[c - 1];
+            }
+            OPS(4);
+        }
+
+    if (adaptive) {
+        /* get_theta_eta_p / _eta_n, scheme_weno5th_SC.f90:330-394 */
+#pragma omp parallel for num_threads(nthreads_of(b)) schedule(static)
+        for (i = 1; i <= ni; i++)
+            for (int jj = 1; jj <= nj + 1; jj++) {
+                double lf[5];
+                for (int m = -2; m <= 2; m++) lf[m + 2] = FE(b->flux_eta, b, jj + m + BF - 1, i, 1);
+                weno5z_omega(lf, &TE(b->theta_etap, b, 2, jj, i), &TE(b->theta_etap, b, 3, jj, i),
+                             &TE(b->theta_etap, b, 4, jj, i), &TE(b->theta_etap, b, 1, jj, i));
+                for (int m = -2; m <= 2; m++) lf[m + 2] = FE(b->flux_eta, b, jj - m + BF, i, 2);
+                weno5z_omega(lf, &TE(b->theta_etan, b, 2, jj, i), &TE(b->theta_etan, b, 3, jj, i),
+                             &TE(b->theta_etan, b, 4, jj, i), &TE(b->theta_etan, b, 1, jj, i));
+            }
+    }
+
+    /* y faces: weno_character_5th_eta_p/_n, scheme_weno5th_CH.f90:176-347;
+       weno_semi_chart_5th_eta_p/_n, scheme_weno5th_SC.f90:396-605 */
+#pragma omp parallel for num_threads(nthreads_of(b)) schedule(static)
+    for (i = 1; i <= ni; i++)
+        for (int jj = 1; jj <= nj + 1; jj++) {
+            double hp[4], hn[4];
+            int cp = 1, cn = 1;
+            if (adaptive) {
+                cp = !(TE(b->theta_etap, b, 1, jj, i) > delta);
+                cn = !(TE(b->theta_etan, b, 1, jj, i) > delta);
+            }
+            double l[16], r[16];
+            if (cp || cn) {
+                int jl = jj + BF - 1, jr = jj + BF, ic = i + BF;
+                double u0, v0, h0, a0;
+                roe_average(gamma, PRIM(b, ic, jl, 1), PRIM(b, ic, jl, 2), PRIM(b, ic, jl, 3), PRIM(b, ic, jl, 4),
+                            PRIM(b, ic, jr, 1), PRIM(b, ic, jr, 2), PRIM(b, ic, jr, 3), PRIM(b, ic, jr, 4),
+                            &u0, &v0, &h0, &a0);
+                lr_vec_eta(u0, v0, h0, a0, gamma, l, r);
+            }
+            double f[5][4];
+            if (cp) {
+                for (int m = -2; m <= 2; m++)
+                    for (int c = 0; c < 4; c++) f[m + 2][c] = FE(b->flux_etap, b, jj + BF - 1 + m, i, c + 1);
+                char_face(l, r, f, hp);
+            } else {
+                double w0 = TE(b->theta_etap, b, 2, jj, i), w1 = TE(b->theta_etap, b, 3, jj, i),
+                       w2 = TE(b->theta_etap, b, 4, jj, i);
+                for (int c = 1; c <= 4; c++) {
+                    double lf[5], q0, q1, q2;
+                    for (int m = -2; m <= 2; m++) lf[m + 2] = FE(b->flux_etap, b, jj + BF - 1 + m, i, c);
+                    candidates(lf, &q0, &q1, &q2);
+                    hp[c - 1] = w0 * q0 + w1 * q1 + w2 * q2;
+                    OPS(5);
+                }
+            }
+            if (cn) {
+                for (int m = -2; m <= 2; m++)
+                    for (int c = 0; c < 4; c++) f[m + 2][c] = FE(b->flux_etan, b, jj + BF - m, i, c + 1);
+                char_face(l, r, f, hn);
+            } else {
+                double w0 = TE(b->theta_etan, b, 2, jj, i), w1 = TE(b->theta_etan, b, 3, jj, i),
+                       w2 = TE(b->theta_etan, b, 4, jj, i);
+                for (int c = 1; c <= 4; c++) {
+                    double lf[5], q0, q1, q2;
+                    for (int m = -2; m <= 2; m++) lf[m + 2] = FE(b->flux_etan, b, jj + BF - m, i, c);
+                    candidates(lf, &q0, &q1, &q2);
+                    hn[c - 1] = w0 * q0 + w1 * q1 + w2 * q2;
+                    OPS(5);
+                }
+            }
+            for (int c = 1; c <= 4; c++) {
+                HE(b->h_etap, b, jj, i, c) = hp[c - 1];
+                HE(b->h_etan, b, jj, i, c) = hn[c - 1];
+                HE(b->h_eta, b, jj, i, c) = hp[c - 1] + hn[c - 1];
+            }
+            OPS(4);
+        }
+}
+
+/* flux_splitting_general, flux_splitting.f90:4-33 */
+static void flux_splitting_general(ora_block* b)
+{
+    switch (b->p.splitting) {
+    case 101: split_fluxes(b, 0, 0); recon_componentwise(b); break;
+    case 102: split_fluxes(b, 1, 0); recon_componentwise(b); break;
+    case 103: split_fluxes(b, 2, 0); recon_componentwise(b); break;
+    case 104: split_fluxes(b, 1, 0); recon_characteristic(b, 0); break;
+    case 105: split_fluxes(b, 1, 1); recon_characteristic(b, 1); break;
+    default: break;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* time_rk3_general, time_advance_general.f90:1-269                    */
+/* ------------------------------------------------------------------ */
+static void time_rk3(ora_block* b, int irk)
+{
+    const int ni = b->p.ni, nj = b->p.nj;
+    const double dxi = b->p.dx, deta = b->p.dy;
+    int j;
+    if (irk == 1) {
+        if (!b->p.if_const_dt) { /* :20-24 */
+            double dt_xi = b->p.dx / b->max_eig_i;
+            double dt_eta = b->p.dy / b->max_eig_j;
+            b->dt = b->p.cfl * dt_xi * dt_eta / (dt_xi + dt_eta);
+        } else {
+            b->dt = b->p.dt_const;
+        }
+        memcpy(b->cons_temp, b->cons, sizeof(double) * (size_t)b->nip * b->njp * NEQ); /* copy_field */
+    }
+    const double dt = b->dt;
+    for (int k = 1; k <= NEQ; k++) {
+#pragma omp parallel for num_threads(nthreads_of(b)) schedule(static)
+        for (j = BF + 1; j <= nj + BF; j++)
+            for (int i = BF + 1; i <= ni + BF; i++) {
+                double dh_dxi = HX(b->h_xi, b, i - BF + 1, j - BF, k) - HX(b->h_xi, b, i - BF, j - BF, k);
+                double dh_deta = HE(b->h_eta, b, j - BF + 1, i - BF, k) - HE(b->h_eta, b, j - BF, i - BF, k);
+                double q;
+                if (irk == 1) {
+                    double q0 = CONS(b, i, j, k);
+                    q = q0 - dt / dxi * dh_dxi - dt / deta * dh_deta; /* :49 */
+                } else if (irk == 2) {
+                    double q0 = CTMP(b, i, j, k), q1 = CONS(b, i, j, k);
+                    q = 3.0 / 4.0 * q0 + 1.0 / 4.0 * q1 - 1.0 / 4.0 * (dt / dxi * dh_dxi + dt / deta * dh_deta);
+                } else {
+                    double q0 = CTMP(b, i, j, k), q1 = CONS(b, i, j, k);
+                    q = 1.0 / 3.0 * q0 + 2.0 / 3.0 * q1 - 2.0 / 3.0 * (dt / dxi * dh_dxi + dt / deta * dh_deta);
+                }
+                CONS(b, i, j, k) = q;
+            }
+    }
+    /* 2 differences + stage 1: 4 (dt/dx counted as invariant) ; stages 2/3: 7 */
+    OPS((long long)ni * nj * NEQ * (irk == 1 ? 6 : 9));
+    if (b->p.if_bodyforce) { /* :71-84, :152-165, :234-247 */
+        double coef = irk == 1 ? dt : (irk == 2 ? 1.0 / 4.0 * dt : SP(2.0 / 3.0) * dt);
+        for (int k = 1; k <= NEQ; k++)
+            for (j = BF + 1; j <= nj + BF; j++)
+                for (int i = BF + 1; i <= ni + BF; i++) CONS(b, i, j, k) = CONS(b, i, j, k) + coef * BFRC(b, i, j, k);
+        OPS((long long)ni * nj * NEQ * 2);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Driver pieces, general_main.f90:23-37                               */
+/* ------------------------------------------------------------------ */
+void ora_step(ora_block* b)
+{
+#ifdef ORA_COUNT_OPS
+    g_ops = 0;
+#endif
+    for (int irk = 1; irk <= b->p.stages; irk++) {
+        get_primary(b);
+        boundary(b);
+        flux_splitting_general(b);
+        time_rk3(b, irk);
+    }
+    b->current_step += 1;
+    b->current_time += b->dt;
+}
+
+int ora_run(ora_block* b, double total_time, int total_step)
+{
+    int n = 0;
+    while (b->current_time < total_time && b->current_step < total_step) {
+        ora_step(b);
+        n++;
+    }
+    return n;
+}
+
+long long ora_op_count(void)
+{
+#ifdef ORA_COUNT_OPS
+    return g_ops;
+#else
+    return -1;
+#endif
+}
+
+/* ------------------------------------------------------------------ */
+/* Allocation (flow_mod_general.f90:190-295) and canned problems       */
+/* ------------------------------------------------------------------ */
+ora_block* ora_create(const ora_params* p)
+{
+    ora_block* b = (ora_block*)calloc(1, sizeof(ora_block));
+    if (!b) return NULL;
+    b->p = *p;
+    if (b->p.stages < 1 || b->p.stages > 3) b->p.stages = 3;
+    int ni = p->ni, nj = p->nj;
+    b->nip = ni + 2 * BF;
+    b->njp = nj + 2 * BF;
+    size_t npad = (size_t)b->nip * b->njp;
+    b->cons = (double*)calloc(npad * NEQ, sizeof(double));
+    b->prim = (double*)calloc(npad * NPRIM, sizeof(double));
+    b->cons_temp = (double*)calloc(npad * NEQ, sizeof(double));
+    b->bodyforce = (double*)calloc(npad * NEQ, sizeof(double));
+    b->flux_xip = (double*)calloc((size_t)b->nip * nj * NEQ, sizeof(double));
+    b->flux_xin = (double*)calloc((size_t)b->nip * nj * NEQ, sizeof(double));
+    b->flux_xi = (double*)calloc((size_t)b->nip * nj * NEQ, sizeof(double));
+    b->flux_etap = (double*)calloc((size_t)b->njp * ni * NEQ, sizeof(double));
+    b->flux_etan = (double*)calloc((size_t)b->njp * ni * NEQ, sizeof(double));
+    b->flux_eta = (double*)calloc((size_t)b->njp * ni * NEQ, sizeof(double));
+    size_t nhx = (size_t)(ni + 1) * nj * NEQ, nhe = (size_t)(nj + 1) * ni * NEQ;
+    b->h_xip = (double*)calloc(nhx, sizeof(double));
+    b->h_xin = (double*)calloc(nhx, sizeof(double));
+    b->h_xi = (double*)calloc(nhx, sizeof(double));
+    b->h_etap = (double*)calloc(nhe, sizeof(double));
+    b->h_etan = (double*)calloc(nhe, sizeof(double));
+    b->h_eta = (double*)calloc(nhe, sizeof(double));
+    b->theta_xip = (double*)calloc(4 * (size_t)(ni + 1) * nj, sizeof(double));
+    b->theta_xin = (double*)calloc(4 * (size_t)(ni + 1) * nj, sizeof(double));
+    b->theta_etap = (double*)calloc(4 * (size_t)(nj + 1) * ni, sizeof(double));
+    b->theta_etan = (double*)calloc(4 * (size_t)(nj + 1) * ni, sizeof(double));
+    b->dt = p->dt_const;
+    return b;
+}
+
+void ora_destroy(ora_block* b)
+{
+    if (!b) return;
+    double* arrs[] = {b->cons, b->prim, b->cons_temp, b->bodyforce, b->flux_xip, b->flux_xin, b->flux_xi,
+                      b->flux_etap, b->flux_etan, b->flux_eta, b->h_xip, b->h_xin, b->h_xi, b->h_etap,
+                      b->h_etan, b->h_eta, b->theta_xip, b->theta_xin, b->theta_etap, b->theta_etan};
+    for (size_t k = 0; k < sizeof(arrs) / sizeof(arrs[0]); k++) free(arrs[k]);
+    free(b);
+}
+
+static void dmr_post_shock(double* s)
+{
+    /* boundary_doublemach_general.f90:47-50: rho=8, u=8.25*sqrt(3)*0.5, v=-8.25*0.5, p=116.5 */
+    s[0] = 8.0;
+    s[1] = 8.25 * sqrt(3.0) * 0.5;
+    s[2] = -8.25 * 0.5;
+    s[3] = 116.5;
+}
+
+int ora_problem_params(int benchmark, int ni, int nj, ora_params* p)
+{
+    p->ni = ni;
+    p->nj = nj;
+    switch (benchmark) {
+    case 1: /* init_IVC_general.f90:13-16; boundary_ivc_general.f90 */
+        p->dx = 10.0 / ni;
+        p->dy = 10.0 / nj;
+        p->bc[0] = p->bc[1] = ORA_BC_EXTRAP;
+        p->bc[2] = p->bc[3] = ORA_BC_PERIODIC;
+        return 0;
+    case 3: /* init_RT_general.f90:9-18; boundary_rt_general.f90 */
+        p->gamma = 5.0 / 3.0;
+        p->if_bodyforce = 1;
+        p->dx = 0.25 / (ni - 1.0);
+        p->dy = 1.0 / (nj - 1.0);
+        p->bc[0] = p->bc[1] = ORA_BC_PERIODIC;
+        p->bc[2] = ORA_BC_DIRICHLET;
+        p->bc_state[2][0] = 2.0; p->bc_state[2][1] = 0.0; p->bc_state[2][2] = 0.0; p->bc_state[2][3] = 1.0;
+        p->bc[3] = ORA_BC_DIRICHLET;
+        p->bc_state[3][0] = 1.0; p->bc_state[3][1] = 0.0; p->bc_state[3][2] = 0.0; p->bc_state[3][3] = 2.5;
+        return 0;
+    case 5: /* init_doublemach_general.f90:13-16; boundary_doublemach_general.f90 */
+        p->dx = 4.0 / ni;
+        p->dy = 1.0 / nj;
+        p->bc[0] = ORA_BC_DIRICHLET;
+        dmr_post_shock(p->bc_state[0]);
+        p->bc[1] = ORA_BC_EXTRAP;
+        p->bc[2] = ORA_BC_DMR_WALL;
+        dmr_post_shock(p->bc_state[2]);
+        p->bc[3] = ORA_BC_DMR_TOP;
+        dmr_post_shock(p->bc_state[3]);
+        return 0;
+    case 9: /* init_shockvortex_general.f90:13-16; boundary_shockvortex_general.f90 */
+        p->dx = 2.0 / ni;
+        p->dy = 1.0 / nj;
+        p->bc[0] = p->bc[1] = ORA_BC_EXTRAP;
+        p->bc[2] = p->bc[3] = ORA_BC_REFLECT;
+        return 0;
+    default:
+        return -1;
+    }
+}
+
+static void put_cell(ora_block* b, int i, int j, double rho, double u, double v, double p)
+{
+    const double gamma = b->p.gamma;
+    int I = i + BF, J = j + BF;
+    CONS(b, I, J, 1) = rho;
+    CONS(b, I, J, 2) = rho * u;
+    CONS(b, I, J, 3) = rho * v;
+    CONS(b, I, J, 4) = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v);
+    PRIM(b, I, J, 1) = rho;
+    PRIM(b, I, J, 2) = u;
+    PRIM(b, I, J, 3) = v;
+    PRIM(b, I, J, 4) = p;
+    PRIM(b, I, J, 5) = sqrt(gamma * p / rho);
+}
+
+int ora_init_state(ora_block* b, int benchmark)
+{
+    const int ni = b->p.ni, nj = b->p.nj;
+    const double gamma = b->p.gamma, dx = b->p.dx, dy = b->p.dy;
+    const double pie = atan(1.0) * 4.0;
+    int i, j;
+    switch (benchmark) {
+    case 1: { /* init_IVC_general.f90:33-56 */
+        const double bb = 0.5;
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) {
+                double x = (double)((float)i - 0.5f) * dx, y = (double)((float)j - 0.5f) * dy;
+                double r = sqrt((x - 5.0) * (x - 5.0) + (y - 5.0) * (y - 5.0));
+                double rho = pow(1.0 - (gamma - 1.0) * (bb * bb) * exp(1 - r * r) / (8.0 * gamma * (pie * pie)), 2.5);
+                double u = 0.0 - bb / (2 * pie) * exp((1 - r * r) / 2) * (y - 5.0);
+                double v = bb / (2 * pie) * exp((1 - r * r) / 2) * (x - 5.0);
+                double p = pow(rho, gamma);
+                put_cell(b, i, j, rho, u, v, p);
+            }
+        return 0;
+    }
+    case 3: { /* init_RT_general.f90:34-65 */
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) {
+                double x = ((double)i - 1.0) * dx, y = ((double)j - 1.0) * dy;
+                double rho, p, u = 0.0, v, c;
+                if (y <= SP(0.5)) {
+                    rho = 2.0;
+                    p = 2.0 * y + 1.0;
+                    c = sqrt(gamma * p / rho);
+                    v = -(0.025 * c * cos(8.0 * pie * x));
+                } else {
+                    rho = 1.0;
+                    p = y + 1.5;
+                    c = sqrt(gamma * p / rho);
+                    v = -(SP(0.025) * c * cos(8.0 * pie * x));
+                }
+                put_cell(b, i, j, rho, u, v, p);
+                BFRC(b, i + BF, j + BF, 1) = 0.0;
+                BFRC(b, i + BF, j + BF, 2) = 0.0;
+                BFRC(b, i + BF, j + BF, 3) = rho;
+                BFRC(b, i + BF, j + BF, 4) = rho * v;
+            }
+        return 0;
+    }
+    case 5: { /* init_doublemach_general.f90:36-59 */
+        double post[4];
+        dmr_post_shock(post);
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) {
+                double x = ((double)i - 0.5) * dx, y = ((double)j - 0.5) * dy;
+                if (x < (1.0 / 6.0 + y * sqrt(1.0 / 3.0)))
+                    put_cell(b, i, j, post[0], post[1], post[2], post[3]);
+                else
+                    put_cell(b, i, j, 1.4, 0.0, 0.0, 1.0);
+            }
+        return 0;
+    }
+    case 9: { /* init_shockvortex_general.f90:34-64 */
+        const double x_c = SP(0.25), y_c = SP(0.5);
+        const double epsl = SP(0.3), r_c = SP(0.05), alpha = SP(0.204);
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) {
+                double x = (double)((float)i - 0.5f) * dx, y = (double)((float)j - 0.5f) * dy;
+                double r = sqrt((x - x_c) * (x - x_c) + (y - y_c) * (y - y_c));
+                double tau = r / r_c;
+                double rho, u, v, p;
+                if (x < SP(0.5)) {
+                    rho = pow(1.0 - (gamma - 1.0) * (epsl * epsl) * exp(2 * alpha * (1 - tau * tau)) / (4 * alpha * gamma),
+                              1.0 / (gamma - 1));
+                    u = SP(1.1) * sqrt(gamma) + epsl * tau * exp(alpha * (1.0 - tau * tau)) * (y - y_c) / r;
+                    v = -(epsl * tau * exp(alpha * (1.0 - tau * tau)) * (x - x_c) / r);
+                    p = pow(rho, gamma);
+                } else {
+                    rho = 1.169082;
+                    u = 1.113299;
+                    v = 0.0;
+                    p = 1.245000;
+                }
+                put_cell(b, i, j, rho, u, v, p);
+            }
+        return 0;
+    }
+    default:
+        return -1;
+    }
+}
+
+void ora_set_cons(ora_block* b, const double* cons)
+{
+    memcpy(b->cons, cons, sizeof(double) * (size_t)b->nip * b->njp * NEQ);
+}
+void ora_get_cons(const ora_block* b, double* cons)
+{
+    memcpy(cons, b->cons, sizeof(double) * (size_t)b->nip * b->njp * NEQ);
+}
+void ora_get_prim(const ora_block* b, double* prim)
+{
+    memcpy(prim, b->prim, sizeof(double) * (size_t)b->nip * b->njp * NPRIM);
+}
+void ora_set_bodyforce(ora_block* b, const double* bf)
+{
+    memcpy(b->bodyforce, bf, sizeof(double) * (size_t)b->nip * b->njp * NEQ);
+}
+double ora_time(const ora_block* b) { return b->current_time; }
+void ora_set_time(ora_block* b, double t) { b->current_time = t; }
+int ora_current_step(const ora_block* b) { return b->current_step; }
+double ora_dt(const ora_block* b) { return b->dt; }
+void ora_set_dt(ora_block* b, double dt) { b->dt = dt; }
+void ora_max_eig(const ora_block* b, double out[2])
+{
+    out[0] = b->max_eig_i;
+    out[1] = b->max_eig_j;
+}
+void ora_update_params(ora_block* b, double gamma, double cfl, double delta, int if_const_dt, double dt_const,
+                       int if_bodyforce)
+{
+    b->p.gamma = gamma;
+    b->p.cfl = cfl;
+    b->p.delta = delta;
+    b->p.if_const_dt = if_const_dt;
+    b->p.dt_const = dt_const;
+    b->p.if_bodyforce = if_bodyforce;
+}
