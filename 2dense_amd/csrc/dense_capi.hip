@@ -1,0 +1,463 @@
+// C ABI of the explicit-update library: one solver handle per block, one HIP
+// stream per handle, device-resident state (three interior buffers for the
+// three RK stages, no ghost layers in HBM) and a device-resident clock.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "dense_internal.h"
+
+using namespace dense;
+
+struct dense_solver {
+    dense_cfg cfg{};
+    int scheme = -1;
+    hipStream_t stream = nullptr;
+    int ld = 0;              // row pitch in doubles
+    long long plane = 0;     // ld * nj
+    double* U[3] = {nullptr, nullptr, nullptr};  // U^n, stage 1, stage 2
+    DevState* st = nullptr;  // device
+    DevState* st_host = nullptr;  // pinned readback
+    double* dt_hist = nullptr;    // device
+    int hist_cap = 0;
+    double* pad_dev = nullptr;    // padded cons + prim for get_state
+    double* pad_host = nullptr;   // pinned staging
+    double* stage_host = nullptr; // pinned interior staging for set_state
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // bracket the last dense_run on the stream
+    double last_run_ms = 0.0;
+    int last_run_launches = 0;
+    int tiles_x = 0, ntiles = 0;
+    char err[512] = {0};
+};
+
+static thread_local char g_err[512];
+
+static int fail(dense_solver* h, int code, const char* fmt, ...)
+{
+    char* buf = h ? h->err : g_err;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, 512, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(h, call)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) return fail(h, DENSE_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+static int validate(const dense_cfg* c, char* why, size_t n)
+{
+    if (c->ni < 4 || c->nj < 4) return snprintf(why, n, "ni, nj must be >= 4 (got %d, %d)", c->ni, c->nj), DENSE_ERR_ARG;
+    if ((long long)c->ni * c->nj > (1ll << 31)) return snprintf(why, n, "grid too large"), DENSE_ERR_ARG;
+    if (c->nghost != 4) return snprintf(why, n, "nghost must be 4 (reference bfsize)"), DENSE_ERR_ARG;
+    if (c->neq != 4) return snprintf(why, n, "neq must be 4"), DENSE_ERR_ARG;
+    if (!(c->dx > 0) || !(c->dy > 0)) return snprintf(why, n, "dx, dy must be positive"), DENSE_ERR_ARG;
+    if (c->stages < 1 || c->stages > 3) return snprintf(why, n, "stages must be 1..3"), DENSE_ERR_ARG;
+    if (scheme_of(c->splitting, c->recon) < 0)
+        return snprintf(why, n, "unsupported switch pair splitting=%d recon=%d", c->splitting, c->recon),
+               DENSE_ERR_UNSUPPORTED;
+    if (c->tile < 0 || c->tile > 3) return snprintf(why, n, "tile index out of range"), DENSE_ERR_ARG;
+    for (int s = 0; s < 4; s++)
+        if (c->bc[s] < 0 || c->bc[s] > 5) return snprintf(why, n, "bad boundary kind on side %d", s), DENSE_ERR_ARG;
+    if ((c->bc[0] == DENSE_BC_PERIODIC) != (c->bc[1] == DENSE_BC_PERIODIC) ||
+        (c->bc[2] == DENSE_BC_PERIODIC) != (c->bc[3] == DENSE_BC_PERIODIC))
+        return snprintf(why, n, "periodic boundaries must be paired"), DENSE_ERR_ARG;
+    if (c->bc[0] == DENSE_BC_DMR_WALL || c->bc[0] == DENSE_BC_DMR_TOP || c->bc[1] == DENSE_BC_DMR_WALL ||
+        c->bc[1] == DENSE_BC_DMR_TOP || c->bc[2] == DENSE_BC_DMR_TOP || c->bc[3] == DENSE_BC_DMR_WALL)
+        return snprintf(why, n, "double-Mach wall/top kinds only on bottom/top"), DENSE_ERR_ARG;
+    const int need = c->ni < c->nj ? c->ni : c->nj;
+    if (need < 4) return snprintf(why, n, "grid must have at least 4 cells per direction"), DENSE_ERR_ARG;
+    return DENSE_OK;
+}
+
+extern "C" int dense_problem_defaults(int benchmark, int ni, int nj, dense_cfg* c)
+{
+    if (!c || ni < 2 || nj < 2) return fail(nullptr, DENSE_ERR_ARG, "bad arguments");
+    c->ni = ni;
+    c->nj = nj;
+    c->nghost = 4;
+    c->neq = 4;
+    c->benchmark = benchmark;
+    if (c->stages == 0) c->stages = 3;
+    const double post[4] = {8.0, 8.25 * std::sqrt(3.0) * 0.5, -8.25 * 0.5, 116.5};
+    switch (benchmark) {
+    case 1:  // init_IVC_general.f90:13-16, boundary_ivc_general.f90
+        c->dx = 10.0 / ni;
+        c->dy = 10.0 / nj;
+        c->bc[0] = c->bc[1] = DENSE_BC_EXTRAP;
+        c->bc[2] = c->bc[3] = DENSE_BC_PERIODIC;
+        return DENSE_OK;
+    case 3:  // init_RT_general.f90:9-18, boundary_rt_general.f90
+        c->gamma = 5.0 / 3.0;
+        c->if_bodyforce = 1;
+        c->dx = 0.25 / (ni - 1.0);
+        c->dy = 1.0 / (nj - 1.0);
+        c->bc[0] = c->bc[1] = DENSE_BC_PERIODIC;
+        c->bc[2] = DENSE_BC_DIRICHLET;
+        c->bc_state[2][0] = 2.0, c->bc_state[2][1] = 0.0, c->bc_state[2][2] = 0.0, c->bc_state[2][3] = 1.0;
+        c->bc[3] = DENSE_BC_DIRICHLET;
+        c->bc_state[3][0] = 1.0, c->bc_state[3][1] = 0.0, c->bc_state[3][2] = 0.0, c->bc_state[3][3] = 2.5;
+        return DENSE_OK;
+    case 5:  // init_doublemach_general.f90:13-16, boundary_doublemach_general.f90
+        c->dx = 4.0 / ni;
+        c->dy = 1.0 / nj;
+        c->bc[0] = DENSE_BC_DIRICHLET;
+        c->bc[1] = DENSE_BC_EXTRAP;
+        c->bc[2] = DENSE_BC_DMR_WALL;
+        c->bc[3] = DENSE_BC_DMR_TOP;
+        for (int s = 0; s < 4; s++)
+            for (int v = 0; v < 4; v++) c->bc_state[s][v] = post[v];
+        return DENSE_OK;
+    case 9:  // init_shockvortex_general.f90:13-16, boundary_shockvortex_general.f90
+        c->dx = 2.0 / ni;
+        c->dy = 1.0 / nj;
+        c->bc[0] = c->bc[1] = DENSE_BC_EXTRAP;
+        c->bc[2] = c->bc[3] = DENSE_BC_REFLECT;
+        return DENSE_OK;
+    case 0:
+        return DENSE_OK;
+    default:
+        return fail(nullptr, DENSE_ERR_UNSUPPORTED, "no such benchmark index %d", benchmark);
+    }
+}
+
+static StageArgs base_args(const dense_solver* h)
+{
+    StageArgs a{};
+    const dense_cfg& c = h->cfg;
+    a.st = h->st;
+    a.dt_hist = h->dt_hist;
+    a.ni = c.ni;
+    a.nj = c.nj;
+    a.ld = h->ld;
+    a.plane = h->plane;
+    a.dx = c.dx;
+    a.dy = c.dy;
+    a.gamma = c.gamma;
+    a.cfl = c.cfl;
+    a.delta = c.delta;
+    a.dt_const = c.dt;
+    a.sqrt3 = std::sqrt(3.0);
+    a.if_const_dt = c.if_const_dt;
+    a.if_bodyforce = c.if_bodyforce;
+    a.tiles_x = h->tiles_x;
+    a.ntiles = h->ntiles;
+    for (int s = 0; s < 4; s++) {
+        a.bc.kind[s] = c.bc[s];
+        for (int v = 0; v < 4; v++) a.bc.state[s][v] = c.bc_state[s][v];
+    }
+    return a;
+}
+
+// Queue one time step: stages 1..S; the last stage writes U^n in place
+// (it reads U^n only at the cell it writes).
+static int enqueue_step(dense_solver* h)
+{
+    StageArgs a = base_args(h);
+    const int S = h->cfg.stages;
+    for (int s = 1; s <= S; s++) {
+        a.stage = s;
+        a.last_stage = (s == S);
+        a.un = h->U[0];
+        a.us = s == 1 ? h->U[0] : h->U[s - 1];
+        a.out = (s == S && S > 1) ? h->U[0] : h->U[s];
+        HIPCHK(h, launch_stage(h->scheme, h->cfg.tile, a, h->stream));
+    }
+    if (S == 1)
+        HIPCHK(h, hipMemcpyAsync(h->U[0], h->U[1], sizeof(double) * 4 * h->plane, hipMemcpyDeviceToDevice, h->stream));
+    return DENSE_OK;
+}
+
+static void release_graph(dense_solver* h)
+{
+    if (h->graph) hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+    h->graph_steps = 0;
+}
+
+extern "C" int dense_create(const dense_cfg* cfg, dense_handle* out)
+{
+    if (!cfg || !out) return fail(nullptr, DENSE_ERR_ARG, "null argument");
+    *out = nullptr;
+    dense_cfg c = *cfg;
+    if (c.benchmark != 0) {
+        dense_cfg d = c;
+        int rc = dense_problem_defaults(c.benchmark, c.ni, c.nj, &d);
+        if (rc) return rc;
+        // metrics/BCs from the canned problem; run switches from the caller
+        for (int s = 0; s < 4; s++) {
+            c.bc[s] = d.bc[s];
+            for (int v = 0; v < 4; v++) c.bc_state[s][v] = d.bc_state[s][v];
+        }
+        if (!(c.dx > 0)) c.dx = d.dx;
+        if (!(c.dy > 0)) c.dy = d.dy;
+    }
+    char why[256];
+    int rc = validate(&c, why, sizeof why);
+    if (rc) return fail(nullptr, rc, "%s", why);
+    dense_solver* h = new (std::nothrow) dense_solver;
+    if (!h) return fail(nullptr, DENSE_ERR_NOMEM, "host allocation failed");
+    h->cfg = c;
+    h->scheme = scheme_of(c.splitting, c.recon);
+    h->ld = (c.ni + 31) / 32 * 32;
+    h->plane = (long long)h->ld * c.nj;
+    h->ntiles = tile_count(c.tile, c.ni, c.nj, &h->tiles_x);
+    auto bail = [&](int code) {
+        dense_destroy(h);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(nullptr, DENSE_ERR_HIP, "hipStreamCreate failed"));
+    for (int k = 0; k < 3; k++)
+        if (hipMalloc(&h->U[k], sizeof(double) * 4 * h->plane) != hipSuccess)
+            return bail(fail(nullptr, DENSE_ERR_NOMEM, "hipMalloc of state failed"));
+    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+        return bail(fail(nullptr, DENSE_ERR_HIP, "hipEventCreate failed"));
+    if (hipMalloc(&h->st, sizeof(DevState)) != hipSuccess || hipHostMalloc(&h->st_host, sizeof(DevState)) != hipSuccess)
+        return bail(fail(nullptr, DENSE_ERR_NOMEM, "state allocation failed"));
+    h->hist_cap = 4096;
+    if (hipMalloc(&h->dt_hist, sizeof(double) * h->hist_cap) != hipSuccess)
+        return bail(fail(nullptr, DENSE_ERR_NOMEM, "dt history allocation failed"));
+    for (int k = 0; k < 3; k++)
+        if (hipMemsetAsync(h->U[k], 0, sizeof(double) * 4 * h->plane, h->stream) != hipSuccess)
+            return bail(fail(nullptr, DENSE_ERR_HIP, "memset failed"));
+    DevState z{};
+    z.hist_cap = h->hist_cap;
+    z.t_end = 0.0;
+    if (hipMemcpyAsync(h->st, &z, sizeof z, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        return bail(fail(nullptr, DENSE_ERR_HIP, "state init failed"));
+    *out = h;
+    return DENSE_OK;
+}
+
+extern "C" int dense_destroy(dense_handle h)
+{
+    if (!h) return DENSE_OK;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    release_graph(h);
+    for (int k = 0; k < 3; k++)
+        if (h->U[k]) hipFree(h->U[k]);
+    if (h->st) hipFree(h->st);
+    if (h->st_host) hipHostFree(h->st_host);
+    if (h->dt_hist) hipFree(h->dt_hist);
+    if (h->pad_dev) hipFree(h->pad_dev);
+    if (h->pad_host) hipHostFree(h->pad_host);
+    if (h->stage_host) hipHostFree(h->stage_host);
+    if (h->ev0) hipEventDestroy(h->ev0);
+    if (h->ev1) hipEventDestroy(h->ev1);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return DENSE_OK;
+}
+
+static int read_state(dense_solver* h)
+{
+    HIPCHK(h, hipMemcpyAsync(h->st_host, h->st, sizeof(DevState), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return DENSE_OK;
+}
+
+extern "C" int dense_set_state(dense_handle h, const double* cons)
+{
+    if (!h || !cons) return fail(h, DENSE_ERR_ARG, "null argument");
+    const int ni = h->cfg.ni, nj = h->cfg.nj, nip = ni + 2 * BF, njp = nj + 2 * BF;
+    if (!h->stage_host && hipHostMalloc(&h->stage_host, sizeof(double) * 4 * h->plane) != hipSuccess)
+        return fail(h, DENSE_ERR_NOMEM, "pinned staging allocation failed");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int k = 0; k < 4; k++)
+        for (int j = 0; j < nj; j++) {
+            const double* src = cons + (size_t)k * nip * njp + (size_t)(j + BF) * nip + BF;
+            double* dst = h->stage_host + k * h->plane + (long long)j * h->ld;
+            std::memcpy(dst, src, sizeof(double) * ni);
+            for (int i = ni; i < h->ld; i++) dst[i] = 0.0;
+        }
+    HIPCHK(h, hipMemcpyAsync(h->U[0], h->stage_host, sizeof(double) * 4 * h->plane, hipMemcpyHostToDevice, h->stream));
+    // fresh reduction ring, then the wave speeds of the uploaded state
+    int rc = read_state(h);
+    if (rc) return rc;
+    DevState z = *h->st_host;
+    std::memset(z.alpha, 0, sizeof z.alpha);
+    z.seq = 0;
+    z.blocks_done = 0;
+    z.error = 0;
+    z.active = 0;
+    HIPCHK(h, hipMemcpyAsync(h->st, &z, sizeof z, hipMemcpyHostToDevice, h->stream));
+    StageArgs a = base_args(h);
+    a.us = h->U[0];
+    HIPCHK(h, launch_max(a, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return DENSE_OK;
+}
+
+extern "C" int dense_get_state(dense_handle h, double* cons, double* prim)
+{
+    if (!h || !cons) return fail(h, DENSE_ERR_ARG, "null argument");
+    const int ni = h->cfg.ni, nj = h->cfg.nj;
+    const size_t np = (size_t)(ni + 2 * BF) * (nj + 2 * BF);
+    if (!h->pad_dev && hipMalloc(&h->pad_dev, sizeof(double) * 9 * np) != hipSuccess)
+        return fail(h, DENSE_ERR_NOMEM, "padded buffer allocation failed");
+    if (!h->pad_host && hipHostMalloc(&h->pad_host, sizeof(double) * 9 * np) != hipSuccess)
+        return fail(h, DENSE_ERR_NOMEM, "pinned padded buffer allocation failed");
+    StageArgs a = base_args(h);
+    a.us = h->U[0];
+    HIPCHK(h, launch_fill_padded(a, h->pad_dev, h->pad_dev + 4 * np, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->pad_host, h->pad_dev, sizeof(double) * 9 * np, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    std::memcpy(cons, h->pad_host, sizeof(double) * 4 * np);
+    if (prim) {
+        std::memcpy(prim, h->pad_host + 4 * np, sizeof(double) * 5 * np);
+        // entropy s = ln p - ln rho on interior cells (get_primary.f90:67)
+        const int nip = ni + 2 * BF;
+        double* s = prim + 5 * np;
+        std::memset(s, 0, sizeof(double) * np);
+        for (int j = BF; j < nj + BF; j++)
+            for (int i = BF; i < ni + BF; i++) {
+                const size_t o = (size_t)j * nip + i;
+                s[o] = std::log(prim[3 * np + o]) - std::log(prim[o]);
+            }
+    }
+    return DENSE_OK;
+}
+
+extern "C" int dense_step(dense_handle h, double current_time, double* dt)
+{
+    if (!h) return fail(h, DENSE_ERR_ARG, "null handle");
+    HIPCHK(h, launch_set_clock(h->st, current_time, INFINITY, 0, 1, h->stream));
+    int rc = enqueue_step(h);
+    if (rc) return rc;
+    rc = read_state(h);
+    if (rc) return rc;
+    if (dt) *dt = h->st_host->dt;
+    if (h->st_host->error) return fail(h, DENSE_ERR_NONFINITE, "non-finite wave speed after step");
+    return DENSE_OK;
+}
+
+static int build_graph(dense_solver* h, int nsteps)
+{
+    release_graph(h);
+    hipGraph_t g = nullptr;
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < nsteps; i++) {
+        int rc = enqueue_step(h);
+        if (rc) {
+            hipStreamEndCapture(h->stream, &g);
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+    }
+    HIPCHK(h, hipStreamEndCapture(h->stream, &g));
+    hipError_t e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(h, DENSE_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    h->graph_steps = nsteps;
+    return DENSE_OK;
+}
+
+extern "C" int dense_run(dense_handle h, int max_steps, double total_time, int total_step, double* t, int* step,
+                         double* dt_hist, int* steps_taken)
+{
+    if (!h || !t || !step || max_steps < 0) return fail(h, DENSE_ERR_ARG, "bad arguments");
+    if (steps_taken) *steps_taken = 0;
+    if (max_steps == 0) return DENSE_OK;
+    const int chunk = 16;
+    if (h->graph_steps != chunk) {
+        int rc = build_graph(h, chunk);
+        if (rc) return rc;
+    }
+    // the device stops itself once the clock passes total_time/total_step
+    int step_end = *step + max_steps < total_step ? *step + max_steps : total_step;
+    HIPCHK(h, launch_set_clock(h->st, *t, total_time, *step, step_end, h->stream));
+    const int launches = (max_steps + chunk - 1) / chunk;
+    HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+    for (int i = 0; i < launches; i++) HIPCHK(h, hipGraphLaunch(h->graph, h->stream));
+    HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+    int rc = read_state(h);
+    if (rc) return rc;
+    const DevState& s = *h->st_host;
+    const int taken = s.step - *step;
+    if (dt_hist && taken > 0) {
+        const int n = taken < h->hist_cap ? taken : h->hist_cap;
+        HIPCHK(h, hipMemcpy(dt_hist, h->dt_hist, sizeof(double) * n, hipMemcpyDeviceToHost));
+    }
+    *t = s.t;
+    *step = s.step;
+    if (steps_taken) *steps_taken = taken;
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_run_ms = ms;
+    h->last_run_launches = taken * h->cfg.stages;
+    if (s.error) return fail(h, DENSE_ERR_NONFINITE, "non-finite wave speed at step %d", s.step);
+    return DENSE_OK;
+}
+
+extern "C" int dense_last_run_gpu_ms(dense_handle h, double* ms, int* stage_launches)
+{
+    if (!h || !ms) return fail(h, DENSE_ERR_ARG, "null argument");
+    *ms = h->last_run_ms;
+    if (stage_launches) *stage_launches = h->last_run_launches;
+    return DENSE_OK;
+}
+
+extern "C" int dense_max_eig(dense_handle h, double out[2])
+{
+    if (!h || !out) return fail(h, DENSE_ERR_ARG, "null argument");
+    int rc = read_state(h);
+    if (rc) return rc;
+    const DevState& s = *h->st_host;
+    const int slot = s.seq & 3;
+    std::memcpy(&out[0], &s.alpha[slot][0], sizeof(double));
+    std::memcpy(&out[1], &s.alpha[slot][1], sizeof(double));
+    return DENSE_OK;
+}
+
+extern "C" int dense_update_params(dense_handle h, const dense_cfg* cfg)
+{
+    if (!h || !cfg) return fail(h, DENSE_ERR_ARG, "null argument");
+    dense_cfg c = h->cfg;
+    c.gamma = cfg->gamma;
+    c.cfl = cfg->cfl;
+    c.delta = cfg->delta;
+    c.dt = cfg->dt;
+    c.if_const_dt = cfg->if_const_dt;
+    c.if_bodyforce = cfg->if_bodyforce;
+    c.splitting = cfg->splitting;
+    c.recon = cfg->recon;
+    char why[256];
+    int rc = validate(&c, why, sizeof why);
+    if (rc) return fail(h, rc, "%s", why);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->cfg = c;
+    h->scheme = scheme_of(c.splitting, c.recon);
+    release_graph(h);  // kernel arguments changed
+    // the wave speeds of the current state depend on gamma
+    int r2 = read_state(h);
+    if (r2) return r2;
+    StageArgs a = base_args(h);
+    a.us = h->U[0];
+    DevState z = *h->st_host;
+    std::memset(z.alpha, 0, sizeof z.alpha);
+    z.seq = 0;
+    z.blocks_done = 0;
+    HIPCHK(h, hipMemcpyAsync(h->st, &z, sizeof z, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, launch_max(a, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return DENSE_OK;
+}
+
+extern "C" int dense_sync(dense_handle h)
+{
+    if (!h) return fail(h, DENSE_ERR_ARG, "null handle");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return DENSE_OK;
+}
+
+extern "C" const char* dense_last_error(dense_handle h) { return h ? h->err : g_err; }

@@ -1,0 +1,295 @@
+// Device-side numerics of the explicit update: primitive recovery, ghost
+// cells, flux splitting, 5th-order reconstructions and the characteristic
+// projection.  Every expression keeps the reference's evaluation order and
+// the library is built with -ffp-contract=off, so results are bitwise equal
+// to the reference's FP64 arithmetic.  Citations are relative to the
+// reference repository root.
+#pragma once
+#include "dense_internal.h"
+
+namespace dense {
+
+// Single-precision literals of the reference, widened as Fortran widens them.
+constexpr double P_FLOOR = (double)1.0e-6f;        // get_primary.f90:58
+constexpr double WENO_SS = (double)1.0e-6f;        // weno_mod.f90:37
+constexpr double TWO_THIRDS_SP = (double)(2.0f / 3.0f);  // time_advance_general.f90:242
+
+// weno_mod.f90:3-14
+constexpr double A300 = 1.0 / 3.0, A301 = -7.0 / 6.0, A302 = 11.0 / 6.0;
+constexpr double A310 = -1.0 / 6.0, A311 = 5.0 / 6.0, A312 = 1.0 / 3.0;
+constexpr double A320 = 1.0 / 3.0, A321 = 5.0 / 6.0, A322 = -1.0 / 6.0;
+constexpr double C30 = 0.1, C31 = 0.6, C32 = 0.3;
+
+enum Recon : int { R_UP5 = 0, R_JS = 1, R_Z = 2 };
+
+// ---------------------------------------------------------------- stencils
+
+// smoothness indicators, weno_mod.f90:152-159 / :175-182
+__device__ __forceinline__ void smooth_ind(const double* lf, double& is0, double& is1, double& is2)
+{
+    double x, y;
+    x = (lf[0] - 2.0 * lf[1]) + lf[2];
+    y = (lf[0] - 4.0 * lf[1]) + 3.0 * lf[2];
+    is0 = 13.0 * (x * x) / 12.0 + (y * y) / 4.0;
+    x = (lf[1] - 2.0 * lf[2]) + lf[3];
+    y = lf[1] - lf[3];
+    is1 = 13.0 * (x * x) / 12.0 + (y * y) / 4.0;
+    x = (lf[2] - 2.0 * lf[3]) + lf[4];
+    y = (3.0 * lf[2] - 4.0 * lf[3]) + lf[4];
+    is2 = 13.0 * (x * x) / 12.0 + (y * y) / 4.0;
+}
+
+// candidate polynomials, weno_mod.f90:57-59
+__device__ __forceinline__ void candidates(const double* lf, double& q0, double& q1, double& q2)
+{
+    q0 = A300 * lf[0] + A301 * lf[1] + A302 * lf[2];
+    q1 = A310 * lf[1] + A311 * lf[2] + A312 * lf[3];
+    q2 = A320 * lf[2] + A321 * lf[3] + A322 * lf[4];
+}
+
+// WENO-Z non-linear weights (+ AdaWENO indicator theta), weno_mod.f90:67-95,168-188
+__device__ __forceinline__ void weno5z_omega(const double* lf, double& w0, double& w1, double& w2, double* theta)
+{
+    double is0, is1, is2;
+    smooth_ind(lf, is0, is1, is2);
+    const double tau5 = fabs(is2 - is0);
+    double t;
+    t = tau5 / (WENO_SS + is0);
+    const double aa0 = C30 * (1.0 + t * t);
+    t = tau5 / (WENO_SS + is1);
+    const double aa1 = C31 * (1.0 + t * t);
+    t = tau5 / (WENO_SS + is2);
+    const double aa2 = C32 * (1.0 + t * t);
+    const double s = aa0 + aa1 + aa2;
+    w0 = aa0 / s;
+    w1 = aa1 / s;
+    w2 = aa2 / s;
+    if (theta) {
+        const double d = s - 1.0;
+        *theta = 1.0 / (1.0 + d * d);
+    }
+}
+
+// weno5z_stencil, weno_mod.f90:50-65
+__device__ __forceinline__ double weno5z(const double* lf)
+{
+    double q0, q1, q2, w0, w1, w2;
+    candidates(lf, q0, q1, q2);
+    weno5z_omega(lf, w0, w1, w2, nullptr);
+    return w0 * q0 + w1 * q1 + w2 * q2;
+}
+
+// weno5js_stencil, weno_mod.f90:98-165
+__device__ __forceinline__ double weno5js(const double* lf)
+{
+    double q0, q1, q2, is0, is1, is2, s;
+    candidates(lf, q0, q1, q2);
+    smooth_ind(lf, is0, is1, is2);
+    s = WENO_SS + is0;
+    const double aa0 = C30 / (s * s);
+    s = WENO_SS + is1;
+    const double aa1 = C31 / (s * s);
+    s = WENO_SS + is2;
+    const double aa2 = C32 / (s * s);
+    const double sum = aa0 + aa1 + aa2;
+    const double w0 = aa0 / sum, w1 = aa1 / sum, w2 = aa2 / sum;
+    return w0 * q0 + w1 * q1 + w2 * q2;
+}
+
+// scheme_upwind5_general.f90:86-90
+__device__ __forceinline__ double upwind5(const double* lf)
+{
+    return (2.0 * lf[0] - 13.0 * lf[1] + 47.0 * lf[2] + 27.0 * lf[3] - 3.0 * lf[4]) / 60.0;
+}
+
+template <int R>
+__device__ __forceinline__ double recon(const double* lf)
+{
+    if constexpr (R == R_UP5) return upwind5(lf);
+    else if constexpr (R == R_JS) return weno5js(lf);
+    else return weno5z(lf);
+}
+
+// ---------------------------------------------------------------- cells
+
+// Primitive variables of one cell, get_primary.f90:52-61.  pr = (u, v, p, c).
+__device__ __forceinline__ void prim_of(double gamma, const double* c, double* pr)
+{
+    const double rho = c[0];
+    const double u = c[1] / rho;
+    const double v = c[2] / rho;
+    double p = (gamma - 1.0) * (c[3] - 0.5 * rho * (u * u + v * v));
+    if (p < 0.0) p = P_FLOOR;
+    pr[0] = u;
+    pr[1] = v;
+    pr[2] = p;
+    pr[3] = sqrt(gamma * p / rho);
+}
+
+// Dirichlet ghost from a primitive state, boundary_doublemach_general.f90:52-61
+__device__ __forceinline__ void from_state(double gamma, double rho, double u, double v, double p, double* c,
+                                           double* pr)
+{
+    c[0] = rho;
+    c[1] = rho * u;
+    c[2] = rho * v;
+    c[3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v);
+    pr[0] = u;
+    pr[1] = v;
+    pr[2] = p;
+    pr[3] = sqrt(gamma * p / rho);
+}
+
+__device__ __forceinline__ void load_interior(const StageArgs& a, const double* __restrict__ U, int gx, int gy,
+                                              double* c, double* pr)
+{
+    const long long o = (long long)gy * a.ld + gx;
+#pragma unroll
+    for (int k = 0; k < 4; k++) c[k] = U[k * a.plane + o];
+    prim_of(a.gamma, c, pr);
+}
+
+// Ghost cell (gx,gy) outside the interior in exactly one direction, k layers
+// deep (k = 1 adjacent).  Reproduces the reference's boundary routines: the
+// ghost conserved state AND the ghost primitive state, which the reference
+// sets independently (e.g. extrapolated separately, boundary_ivc_general.f90:72-80).
+__device__ __noinline__ void ghost_cell(const StageArgs& a, const double* __restrict__ U, int gx, int gy, double t,
+                                        double* c, double* pr)
+{
+    const int ni = a.ni, nj = a.nj;
+    const bool xside = (gx < 0 || gx >= ni);
+    int side, k;
+    if (xside) {
+        side = gx < 0 ? DENSE_SIDE_LEFT : DENSE_SIDE_RIGHT;
+        k = gx < 0 ? -gx : gx - ni + 1;
+    } else {
+        side = gy < 0 ? DENSE_SIDE_BOTTOM : DENSE_SIDE_TOP;
+        k = gy < 0 ? -gy : gy - nj + 1;
+    }
+    const bool low = (side == DENSE_SIDE_LEFT || side == DENSE_SIDE_BOTTOM);
+    const int n = xside ? ni : nj;
+    int kind = a.bc.kind[side];
+    const double* sv = a.bc.state[side];
+
+    if (kind == DENSE_BC_DMR_WALL) {
+        // x of the cell column, x(index_i-bfsize_i,1) = (i - 0.5d0)*dx
+        const double x = ((double)(gx + 1) - 0.5) * a.dx;
+        kind = (x >= 1.0 / 6.0) ? DENSE_BC_REFLECT : DENSE_BC_DIRICHLET;
+    }
+    switch (kind) {
+    case DENSE_BC_PERIODIC: {
+        const int src = low ? n - k : k - 1;
+        if (xside) load_interior(a, U, src, gy, c, pr);
+        else load_interior(a, U, gx, src, c, pr);
+        return;
+    }
+    case DENSE_BC_REFLECT: {
+        const int src = low ? k - 1 : n - k;
+        if (xside) {
+            load_interior(a, U, src, gy, c, pr);
+            c[1] = -c[1];
+            pr[0] = -pr[0];
+        } else {
+            load_interior(a, U, gx, src, c, pr);
+            c[2] = -c[2];
+            pr[1] = -pr[1];
+        }
+        return;
+    }
+    case DENSE_BC_EXTRAP: {
+        // ghost(k) = ghost(k-1)*4/3 - ghost(k-2)/3, starting from the two
+        // boundary cells (boundary_ivc_general.f90:64-67, loop :63 outward)
+        const int i0 = low ? 0 : n - 1, i1 = low ? 1 : n - 2;
+        double q1[8], q2[8];
+        if (xside) {
+            load_interior(a, U, i0, gy, q1, q1 + 4);
+            load_interior(a, U, i1, gy, q2, q2 + 4);
+        } else {
+            load_interior(a, U, gx, i0, q1, q1 + 4);
+            load_interior(a, U, gx, i1, q2, q2 + 4);
+        }
+        for (int s = 0; s < k; s++) {
+#pragma unroll
+            for (int v = 0; v < 8; v++) {
+                const double g = q1[v] * 4.0 / 3.0 - q2[v] / 3.0;
+                q2[v] = q1[v];
+                q1[v] = g;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            c[v] = q1[v];
+            pr[v] = q1[4 + v];
+        }
+        return;
+    }
+    case DENSE_BC_DMR_TOP: {
+        // boundary_doublemach_general.f90:67-80, shock at t^n
+        const double x = ((double)(gx + 1) - 0.5) * a.dx;
+        if (x < (1.0 / 6.0 + (1.0 + 20.0 * t) / a.sqrt3))
+            from_state(a.gamma, sv[0], sv[1], sv[2], sv[3], c, pr);
+        else
+            from_state(a.gamma, 1.4, 0.0, 0.0, 1.0, c, pr);
+        return;
+    }
+    default:  // DIRICHLET
+        from_state(a.gamma, sv[0], sv[1], sv[2], sv[3], c, pr);
+        return;
+    }
+}
+
+// ---------------------------------------------------------------- fluxes
+
+// Lax-Friedrichs split of one cell along DIR, sign +1/-1:
+// F = 0.5*((u_n +/- alpha)*U + (0, p e_n, u_n p)),
+// flux_splitting_LLF.f90:39-47 (x), :64-72 (y); GLF flux_splitting_GLF.f90:31-43.
+template <int DIR>
+__device__ __forceinline__ void lf_flux(double upa, double un, double p, const double* c, double* f)
+{
+    constexpr int NRM = DIR == 0 ? 1 : 2;
+    f[0] = 0.5 * (upa * c[0]);
+    f[NRM] = 0.5 * (upa * c[NRM] + p);
+    f[3 - NRM] = 0.5 * (upa * c[3 - NRM]);
+    f[3] = 0.5 * (upa * c[3] + un * p);
+}
+
+// Steger-Warming split of one cell along DIR, both signs,
+// flux_splitting_SW.f90:36-92 (x), :103-161 (y).
+template <int DIR>
+__device__ __forceinline__ void sw_flux(double gamma, double rho, double u, double v, double alpha, double* fp,
+                                        double* fm)
+{
+    const double epsl = 0.125;
+    const double un = DIR == 0 ? u : v;
+    const double uu2 = u * u + v * v;
+    const double lam[3] = {un, un - alpha, un + alpha};
+#pragma unroll
+    for (int sgn = 0; sgn < 2; sgn++) {
+        double* f = sgn == 0 ? fp : fm;
+        double lp[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const double sq = sqrt(lam[q] * lam[q] + epsl * epsl);
+            lp[q] = sgn == 0 ? (lam[q] + sq) / 2.0 : (lam[q] - sq) / 2.0;
+        }
+        const double wii = (3.0 - gamma) * (lp[1] + lp[2]) * (alpha * alpha) / (2.0 * (gamma - 1.0));
+        f[0] = (lp[0] * 2.0 * (gamma - 1.0) + lp[1] + lp[2]) * rho / (2.0 * gamma);
+        if (DIR == 0) {
+            f[1] = (lp[0] * 2.0 * (gamma - 1.0) * u + lp[1] * (u - alpha) + lp[2] * (u + alpha)) * rho /
+                   (2.0 * gamma);
+            f[2] = (lp[0] * 2.0 * (gamma - 1.0) * v + lp[1] * v + lp[2] * v) * rho / (2.0 * gamma);
+            f[3] = (lp[0] * (gamma - 1.0) * uu2 + lp[1] * 0.5 * ((u - alpha) * (u - alpha) + v * v) +
+                    lp[2] * 0.5 * ((u + alpha) * (u + alpha) + v * v) + wii) *
+                   rho / (2.0 * gamma);
+        } else {
+            f[1] = (lp[0] * 2.0 * (gamma - 1.0) * u + lp[1] * u + lp[2] * u) * rho / (2.0 * gamma);
+            f[2] = (lp[0] * 2.0 * (gamma - 1.0) * v + lp[1] * (v - alpha) + lp[2] * (v + alpha)) * rho /
+                   (2.0 * gamma);
+            f[3] = (lp[0] * (gamma - 1.0) * uu2 + lp[1] * 0.5 * ((v - alpha) * (v - alpha) + u * u) +
+                    lp[2] * 0.5 * ((v + alpha) * (v + alpha) + u * u) + wii) *
+                   rho / (2.0 * gamma);
+        }
+    }
+}
+
+}  // namespace dense

@@ -1,0 +1,219 @@
+// Face reconstructions of one tile.  A face thread reads the 6 cells of its
+// stencil (f-3 .. f+2 along the sweep direction) from the LDS tile record and
+// produces h(face) = h+(face) + h-(face) for the 4 conserved components,
+// exactly as the reference's sweeps followed by sum_field produce h_xi/h_eta.
+#pragma once
+#include "dense_device.h"
+
+namespace dense {
+
+// LDS record slots of one cell: 0..3 conserved (rho, rho u, rho v, E), then u, v, p, c.
+enum { REC_U = 4, REC_V = 5, REC_P = 6, REC_C = 7, REC_N = 8 };
+
+// Component-wise reconstruction with Lax-Friedrichs splitting.
+//   LLF: alpha = |u_n|+c per cell (flux_splitting_LLF.f90:37);
+//   GLF: alpha = global max (flux_splitting_GLF.f90:31-43).
+// Stencils: + side lf(m) = F+(f-1+m), - side lf(m) = F-(f-m)
+// (scheme_weno5z_general.f90:81, :106).
+template <int DIR, bool LOCAL, int RECON, int NC, int ST>
+__device__ __forceinline__ void face_cw_lf(const double* __restrict__ s, int base, double alpha, double* h)
+{
+    constexpr int NRM = DIR == 0 ? 1 : 2;
+    constexpr int VN = DIR == 0 ? REC_U : REC_V;
+    double up[6], um[6], pp[6], unp[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        const int o = base + m * ST;
+        const double un = s[VN * NC + o], p = s[REC_P * NC + o];
+        const double a = LOCAL ? fabs(un) + s[REC_C * NC + o] : alpha;
+        up[m] = un + a;
+        um[m] = un - a;
+        pp[m] = p;
+        unp[m] = un * p;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double lfp[5], lfm[5];
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            const int ip = i, im = 5 - i;
+            const double qp = s[k * NC + base + ip * ST], qm = s[k * NC + base + im * ST];
+            if (k == NRM) {
+                lfp[i] = 0.5 * (up[ip] * qp + pp[ip]);
+                lfm[i] = 0.5 * (um[im] * qm + pp[im]);
+            } else if (k == 3) {
+                lfp[i] = 0.5 * (up[ip] * qp + unp[ip]);
+                lfm[i] = 0.5 * (um[im] * qm + unp[im]);
+            } else {
+                lfp[i] = 0.5 * (up[ip] * qp);
+                lfm[i] = 0.5 * (um[im] * qm);
+            }
+        }
+        h[k] = recon<RECON>(lfp) + recon<RECON>(lfm);
+    }
+}
+
+// Component-wise reconstruction with Steger-Warming splitting.
+template <int DIR, int RECON, int NC, int ST>
+__device__ __forceinline__ void face_cw_sw(const double* __restrict__ s, int base, double gamma, double* h)
+{
+    double fp[6][4], fm[6][4];
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        const int o = base + m * ST;
+        sw_flux<DIR>(gamma, s[o], s[REC_U * NC + o], s[REC_V * NC + o], s[REC_C * NC + o], fp[m], fm[m]);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double lfp[5], lfm[5];
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            lfp[i] = fp[i][k];
+            lfm[i] = fm[5 - i][k];
+        }
+        h[k] = recon<RECON>(lfp) + recon<RECON>(lfm);
+    }
+}
+
+// Characteristic WENO-Z (switch 104) and AdaWENO (switch 105) at one face.
+//   Roe average of cells f-1, f      scheme_weno5th_CH.f90:44-62
+//   eigenvectors L, R                scheme_weno5th_SC.f90:277-328 (x), :607-658 (y)
+//   projection, WENO-Z, back-proj.   scheme_weno5th_CH.f90:64-81
+//   AdaWENO indicator rho+F_n, theta scheme_weno5th_SC.f90:1-64, flux_splitting_SC.f90:58-59
+//   shared-weight branch (theta>delta) scheme_weno5th_SC.f90:110-124
+// The eigenvectors are written in the frame (normal, tangential); the
+// projection sums run over the conserved components in their original order
+// (rho, rho u, rho v, E) so each sum rounds as the reference's does.
+template <int DIR, bool ADAPT, int NC, int ST>
+__device__ __forceinline__ void face_char(const double* __restrict__ s, int base, double alpha, double gamma,
+                                          double delta, double* h)
+{
+    constexpr int NRM = DIR == 0 ? 1 : 2, TAN = 3 - NRM;
+    constexpr int VN = DIR == 0 ? REC_U : REC_V;
+
+    // --- AdaWENO indicator weights per sign
+    bool chp = true, chm = true;
+    double wp0 = 0, wp1 = 0, wp2 = 0, wm0 = 0, wm1 = 0, wm2 = 0;
+    if constexpr (ADAPT) {
+        double lfi[5], th;
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            const int o = base + i * ST;
+            const double un = s[VN * NC + o];
+            lfi[i] = s[o] + 0.5 * ((un + alpha) * s[NRM * NC + o] + s[REC_P * NC + o]);
+        }
+        weno5z_omega(lfi, wp0, wp1, wp2, &th);
+        chp = !(th > delta);
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            const int o = base + (5 - i) * ST;
+            const double un = s[VN * NC + o];
+            lfi[i] = s[o] + 0.5 * ((un - alpha) * s[NRM * NC + o] + s[REC_P * NC + o]);
+        }
+        weno5z_omega(lfi, wm0, wm1, wm2, &th);
+        chm = !(th > delta);
+    }
+
+    // --- Roe state and eigenvectors (only when a sign needs them)
+    double Lr[3], Ln[3], Lt[3], Le[3];  // chars 1,2,4: entries for rho, n, t, E
+    double ut0 = 0, un0 = 0, a0 = 0, h0 = 0, uu0 = 0;
+    if (chp || chm) {
+        const int oL = base + 2 * ST, oR = base + 3 * ST;
+        const double rhol = s[oL], ul = s[REC_U * NC + oL], vl = s[REC_V * NC + oL], pl = s[REC_P * NC + oL];
+        const double rhor = s[oR], ur = s[REC_U * NC + oR], vr = s[REC_V * NC + oR], pr = s[REC_P * NC + oR];
+        const double hl = pl / (gamma - 1) / rhol + 0.5 * (ul * ul + vl * vl) + pl / rhol;
+        const double hr = pr / (gamma - 1) / rhor + 0.5 * (ur * ur + vr * vr) + pr / rhor;
+        const double sr = sqrt(rhor), sl = sqrt(rhol);
+        const double u0 = (sr * ur + sl * ul) / (sr + sl);
+        const double v0 = (sr * vr + sl * vl) / (sr + sl);
+        h0 = (sr * hr + sl * hl) / (sr + sl);
+        a0 = sqrt((gamma - 1.0) * (h0 - 0.5 * (u0 * u0 + v0 * v0)));
+        un0 = DIR == 0 ? u0 : v0;
+        ut0 = DIR == 0 ? v0 : u0;
+        uu0 = 0.5 * (u0 * u0 + v0 * v0);
+        const double hh0 = a0 * a0 / (gamma - 1.0);
+        const double A = 0.5 * uu0 / hh0, B = 0.5 * un0 / a0, Cc = 0.5 * un0 / hh0, D = 0.5 / a0;
+        const double E = 0.5 * ut0 / hh0, F = 0.5 / hh0;
+        Lr[0] = A + B;        Ln[0] = -Cc - D;     Lt[0] = -E;          Le[0] = F;
+        Lr[1] = -(uu0 / hh0) + 1.0;  Ln[1] = un0 / hh0;  Lt[1] = ut0 / hh0;  Le[1] = -(1.0 / hh0);
+        Lr[2] = A - B;        Ln[2] = -Cc + D;     Lt[2] = -E;          Le[2] = F;
+    }
+
+#pragma unroll
+    for (int sg = 0; sg < 2; sg++) {
+        const bool plus = sg == 0;
+        const bool ch = plus ? chp : chm;
+        double* hs = h + 4 * sg;  // h[0..3] plus, h[4..7] minus
+        // 5-point flux stencil of this sign
+        double F[5][4];
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            const int o = base + (plus ? i : 5 - i) * ST;
+            const double un = s[VN * NC + o];
+            const double upa = plus ? un + alpha : un - alpha;
+            const double c4[4] = {s[o], s[NC + o], s[2 * NC + o], s[3 * NC + o]};
+            lf_flux<DIR>(upa, un, s[REC_P * NC + o], c4, F[i]);
+        }
+        if (ch) {
+            double acc_r = 0, acc_n = 0, acc_t = 0, acc_e = 0;
+#pragma unroll
+            for (int kc = 0; kc < 4; kc++) {
+                double lf[5];
+                if (kc == 2) {
+                    // l(:,3) = (-u_t, [0|1], [1|0], 0): (-u_t)*F_rho + F_t
+#pragma unroll
+                    for (int i = 0; i < 5; i++) lf[i] = (-ut0) * F[i][0] + F[i][TAN];
+                } else {
+                    const int q = kc == 3 ? 2 : kc;
+                    double l4[4];
+                    l4[0] = Lr[q];
+                    l4[NRM] = Ln[q];
+                    l4[TAN] = Lt[q];
+                    l4[3] = Le[q];
+#pragma unroll
+                    for (int i = 0; i < 5; i++)
+                        lf[i] = l4[0] * F[i][0] + l4[1] * F[i][1] + l4[2] * F[i][2] + l4[3] * F[i][3];
+                }
+                const double cv = weno5z(lf);
+                // back-projection h(c) = sum_k r(k,c) cv_k, accumulated in k order
+                if (kc == 0) {
+                    acc_r = cv;
+                    acc_n = (un0 - a0) * cv;
+                    acc_t = ut0 * cv;
+                    acc_e = (h0 - a0 * un0) * cv;
+                } else if (kc == 1) {
+                    acc_r = acc_r + cv;
+                    acc_n = acc_n + un0 * cv;
+                    acc_t = acc_t + ut0 * cv;
+                    acc_e = acc_e + uu0 * cv;
+                } else if (kc == 2) {
+                    acc_t = acc_t + cv;
+                    acc_e = acc_e + ut0 * cv;
+                } else {
+                    acc_r = acc_r + cv;
+                    acc_n = acc_n + (un0 + a0) * cv;
+                    acc_t = acc_t + ut0 * cv;
+                    acc_e = acc_e + (h0 + a0 * un0) * cv;
+                }
+            }
+            hs[0] = acc_r;
+            hs[NRM] = acc_n;
+            hs[TAN] = acc_t;
+            hs[3] = acc_e;
+        } else {
+            const double w0 = plus ? wp0 : wm0, w1 = plus ? wp1 : wm1, w2 = plus ? wp2 : wm2;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                double lf[5], q0, q1, q2;
+#pragma unroll
+                for (int i = 0; i < 5; i++) lf[i] = F[i][k];
+                candidates(lf, q0, q1, q2);
+                hs[k] = w0 * q0 + w1 * q1 + w2 * q2;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) h[k] = h[k] + h[4 + k];
+}
+
+}  // namespace dense

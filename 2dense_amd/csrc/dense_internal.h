@@ -1,0 +1,79 @@
+// Internal structures shared by the kernels and the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include "../../include/dense_capi.h"
+
+namespace dense {
+
+constexpr int BF = 4;  // ghost layers of the reference layout
+
+enum Scheme : int {
+    S_LLF_UP5 = 0, S_LLF_JS, S_LLF_Z,
+    S_GLF_UP5, S_GLF_JS, S_GLF_Z,
+    S_SW_UP5, S_SW_JS, S_SW_Z,
+    S_CH, S_SC,
+    S_COUNT
+};
+
+// Map the reference's (splitting, recon) switch pair to a kernel scheme, or -1.
+inline int scheme_of(int splitting, int recon)
+{
+    int r = recon == 251 ? 0 : recon == 253 ? 1 : recon == 255 ? 2 : -1;
+    switch (splitting) {
+    case 101: return r < 0 ? -1 : S_LLF_UP5 + r;
+    case 102: return r < 0 ? -1 : S_GLF_UP5 + r;
+    case 103: return r < 0 ? -1 : S_SW_UP5 + r;
+    case 104: return S_CH;  // always WENO-Z (scheme_weno5th_CH.f90:73)
+    case 105: return S_SC;
+    default: return -1;
+    }
+}
+
+// Device-resident clock and reduction slots of one solver (or one batch instance).
+struct alignas(64) DevState {
+    unsigned long long alpha[4][2];  // ring of max(|u|+c), max(|v|+c) bit patterns
+    double t;                        // t^n
+    double dt;                       // step size of the current step
+    double t_end;                    // run while t < t_end ...
+    int step;                        // ... and step < step_end
+    int step_end;
+    int seq;                         // kernel sequence number: reads alpha[seq&3], writes alpha[(seq+1)&3]
+    int active;                      // 1 while the current step runs
+    unsigned int blocks_done;        // last-block detection
+    int error;                       // 1: non-finite wave speed
+    int hist_n;                      // dt history entries written
+    int hist_cap;
+};
+
+struct BcDesc {
+    int kind[4];
+    double state[4][4];  // rho, u, v, p
+};
+
+// Arguments of one RK-stage launch (passed by value).
+struct StageArgs {
+    const double* un;   // U^n      (4 planes, pitch ld, nj rows)
+    const double* us;   // stage input U^(s-1)
+    double* out;        // stage output
+    DevState* st;
+    double* dt_hist;
+    int ni, nj, ld;
+    long long plane;    // ld * nj
+    double dx, dy, gamma, cfl, delta, dt_const, sqrt3;
+    int if_const_dt, if_bodyforce;
+    int stage;          // 1..3
+    int last_stage;
+    int tiles_x, ntiles;
+    BcDesc bc;
+};
+
+// host-side launchers (dense_kernels.hip)
+hipError_t launch_stage(int scheme, int tile, const StageArgs& a, hipStream_t s);
+hipError_t launch_max(const StageArgs& a, hipStream_t s);
+hipError_t launch_set_clock(DevState* st, double t, double t_end, int step, int step_end, hipStream_t s);
+hipError_t launch_fill_padded(const StageArgs& a, double* cons_pad, double* prim_pad, hipStream_t s);
+int tile_count(int tile, int ni, int nj, int* tiles_x);
+
+}  // namespace dense

@@ -1,0 +1,379 @@
+// Fused RK-stage kernel of the explicit update and its helpers (gfx950).
+//
+// One workgroup owns a TX x TY tile of interior cells.  Per stage it
+//   1. stages the tile plus a 3-cell halo of conserved variables in LDS with
+//      coalesced loads along i, deriving primitives on the fly
+//      (get_primary.f90:46-67) and synthesising ghost cells from the boundary
+//      rules (boundary_*_general.f90) so no ghost layer lives in HBM;
+//   2. reconstructs all x faces, then all y faces of the tile (one face per
+//      thread), each face producing h = h+ + h- (flux_splitting_*.f90,
+//      scheme_*.f90, field_op.f90:1-21);
+//   3. applies the SSP-RK3 stage update (time_advance_general.f90) and
+//      reduces max|u|+c, max|v|+c of the new state with wavefront shuffles
+//      for the next stage's global Lax-Friedrichs speed and CFL step.
+// Nothing here is an MFMA contraction; the kernel is FP64-VALU bound.
+#include "dense_faces.h"
+
+namespace dense {
+
+template <int TX_, int TY_, int NT_>
+struct Tile {
+    static constexpr int TX = TX_, TY = TY_, NT = NT_;
+    static constexpr int W = TX + 6, H = TY + 6, NC = W * H;
+    static constexpr int NXF = (TX + 1) * TY, NYF = TX * (TY + 1);
+    static_assert(NXF <= NT && NYF <= NT, "one face per thread per direction");
+    static_assert(4 * (NXF + NYF) <= REC_N * NC, "h arrays alias the cell record");
+};
+
+// Tile shapes (index = dense_cfg.tile).  Chosen so that each direction has
+// at most one face per thread and a multiple of 64 threads.
+#define DENSE_TILE_LIST(X) \
+    X(0, 16, 15, 256)      \
+    X(1, 32, 7, 256)       \
+    X(2, 8, 7, 64)         \
+    X(3, 32, 15, 512)
+
+constexpr int NTILES_SHAPES = 4;
+
+__host__ __device__ inline void tile_shape(int tile, int& tx, int& ty, int& nt)
+{
+    switch (tile) {
+#define X(i, a, b, c) \
+    case i: tx = a; ty = b; nt = c; return;
+        DENSE_TILE_LIST(X)
+#undef X
+    default: tx = 16; ty = 15; nt = 256; return;
+    }
+}
+
+// Block -> tile map that hands each XCD a contiguous run of tiles (workgroups
+// are dealt round-robin over the 8 XCDs), so halo re-reads hit the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int n)
+{
+    const int q = n >> 3, r = n & 7;
+    const int x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
+__device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// Block-wide max of (ei, ej) bit patterns into the out slot, skipping the
+// atomic when the slot already holds a larger value.
+template <int NT>
+__device__ __forceinline__ void block_max_to_slot(unsigned long long ei, unsigned long long ej, unsigned long long* slot)
+{
+    __shared__ unsigned long long red[2][NT / 64];
+    ei = wave_max_u64(ei);
+    ej = wave_max_u64(ej);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][w] = ei;
+        red[1][w] = ej;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long mi = red[0][0], mj = red[1][0];
+#pragma unroll
+        for (int k = 1; k < NT / 64; k++) {
+            mi = red[0][k] > mi ? red[0][k] : mi;
+            mj = red[1][k] > mj ? red[1][k] : mj;
+        }
+        if (mi > __hip_atomic_load(&slot[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&slot[0], mi);
+        if (mj > __hip_atomic_load(&slot[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&slot[1], mj);
+    }
+}
+
+// The last workgroup of a launch advances the kernel sequence number, clears
+// the slot the next launch reduces into, checks the new wave speeds and, at
+// the end of a time step, advances the device clock (general_main.f90:35-37).
+__device__ void launch_epilogue(DevState* st, int seq, bool end_of_step, double t, double dt, double* dt_hist)
+{
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    __threadfence();
+    const unsigned int prev =
+        __hip_atomic_fetch_add(&st->blocks_done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev != gridDim.x - 1) return;
+    __threadfence();
+    st->blocks_done = 0;
+    const int out = (seq + 1) & 3, nxt = (seq + 2) & 3;
+    st->alpha[nxt][0] = 0ull;
+    st->alpha[nxt][1] = 0ull;
+    const unsigned long long ai = __hip_atomic_load(&st->alpha[out][0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long aj = __hip_atomic_load(&st->alpha[out][1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long inf_bits = 0x7ff0000000000000ull;
+    if (ai >= inf_bits || aj >= inf_bits) st->error = 1;
+    st->seq = seq + 1;
+    if (end_of_step) {
+        if (dt_hist && st->hist_n < st->hist_cap) dt_hist[st->hist_n] = dt;
+        st->hist_n += 1;
+        st->t = t + dt;
+        st->step += 1;
+        st->active = (st->t < st->t_end) && (st->step < st->step_end) && !st->error;
+    }
+    __threadfence();
+}
+
+// CFL step size, time_advance_general.f90:20-24
+__device__ __forceinline__ double cfl_dt(const StageArgs& a, double ai, double aj)
+{
+    if (a.if_const_dt) return a.dt_const;
+    const double dt_xi = a.dx / ai;
+    const double dt_eta = a.dy / aj;
+    return a.cfl * dt_xi * dt_eta / (dt_xi + dt_eta);
+}
+
+template <int SCHEME, int DIR, int NC, int ST>
+__device__ __forceinline__ void eval_face(const double* __restrict__ s, int base, const StageArgs& a, double alpha,
+                                          double* h)
+{
+    if constexpr (SCHEME == S_CH) face_char<DIR, false, NC, ST>(s, base, alpha, a.gamma, a.delta, h);
+    else if constexpr (SCHEME == S_SC) face_char<DIR, true, NC, ST>(s, base, alpha, a.gamma, a.delta, h);
+    else if constexpr (SCHEME >= S_SW_UP5) face_cw_sw<DIR, SCHEME - S_SW_UP5, NC, ST>(s, base, a.gamma, h);
+    else if constexpr (SCHEME >= S_GLF_UP5) face_cw_lf<DIR, false, SCHEME - S_GLF_UP5, NC, ST>(s, base, alpha, h);
+    else face_cw_lf<DIR, true, SCHEME - S_LLF_UP5, NC, ST>(s, base, alpha, h);
+}
+
+template <int SCHEME, class T>
+__global__ void __launch_bounds__(T::NT) stage_kernel(StageArgs a)
+{
+    constexpr int TX = T::TX, TY = T::TY, NT = T::NT, W = T::W, NC = T::NC;
+    __shared__ double s[REC_N * NC];
+    DevState* st = a.st;
+    if (!st->active) return;
+    const int seq = st->seq;
+    const double ai = __longlong_as_double((long long)st->alpha[seq & 3][0]);
+    const double aj = __longlong_as_double((long long)st->alpha[seq & 3][1]);
+    double dt;
+    if (a.stage == 1) {
+        dt = cfl_dt(a, ai, aj);
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->dt = dt;
+    } else {
+        dt = st->dt;
+    }
+    const double t = st->t;
+
+    const int tile = xcd_tile(blockIdx.x, a.ntiles);
+    const int x0 = (tile % a.tiles_x) * TX, y0 = (tile / a.tiles_x) * TY;
+    const int ni = a.ni, nj = a.nj;
+
+    // ---- 1. tile + halo -> LDS (corners and cells beyond the 3rd ghost layer are never read)
+    for (int idx = threadIdx.x; idx < NC; idx += NT) {
+        const int ly = idx / W, lx = idx - ly * W;
+        const int gx = x0 + lx - 3, gy = y0 + ly - 3;
+        const bool inx = (unsigned)gx < (unsigned)ni, iny = (unsigned)gy < (unsigned)nj;
+        if ((!inx && !iny) || gx >= ni + 3 || gy >= nj + 3) continue;
+        double c[4], pr[4];
+        if (inx && iny) load_interior(a, a.us, gx, gy, c, pr);
+        else ghost_cell(a, a.us, gx, gy, t, c, pr);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            s[k * NC + idx] = c[k];
+            s[(4 + k) * NC + idx] = pr[k];
+        }
+    }
+    __syncthreads();
+
+    // ---- 2. faces: x faces (fx in [0,TX], ly in [0,TY)), then y faces
+    double hx[8], hy[8];
+    const int tid = threadIdx.x;
+    const int fx = tid % (TX + 1), lyx = tid / (TX + 1);
+    const bool vx = tid < T::NXF && x0 + fx <= ni && y0 + lyx < nj;
+    if (vx) eval_face<SCHEME, 0, NC, 1>(s, (lyx + 3) * W + fx, a, ai, hx);
+    const int lxy = tid % TX, fy = tid / TX;
+    const bool vy = tid < T::NYF && x0 + lxy < ni && y0 + fy <= nj;
+    if (vy) eval_face<SCHEME, 1, NC, W>(s, fy * W + lxy + 3, a, aj, hy);
+    __syncthreads();
+    double* shx = s;                         // [4][TY][TX+1]
+    double* shy = s + 4 * T::NXF;            // [4][TY+1][TX]
+    if (vx) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) shx[k * T::NXF + lyx * (TX + 1) + fx] = hx[k];
+    }
+    if (vy) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) shy[k * T::NYF + fy * TX + lxy] = hy[k];
+    }
+    __syncthreads();
+
+    // ---- 3. RK stage update, time_advance_general.f90:35-54 / :113-135 / :195-217
+    unsigned long long ei = 0, ej = 0;
+    for (int idx = tid; idx < TX * TY; idx += NT) {
+        const int lx = idx % TX, ly = idx / TX;
+        const int gx = x0 + lx, gy = y0 + ly;
+        if (gx >= ni || gy >= nj) continue;
+        const long long o = (long long)gy * a.ld + gx;
+        double q[4], qs[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double dh_dxi = shx[k * T::NXF + ly * (TX + 1) + lx + 1] - shx[k * T::NXF + ly * (TX + 1) + lx];
+            const double dh_deta = shy[k * T::NYF + (ly + 1) * TX + lx] - shy[k * T::NYF + ly * TX + lx];
+            const double q1 = a.us[k * a.plane + o];
+            qs[k] = q1;
+            if (a.stage == 1) {
+                q[k] = q1 - dt / a.dx * dh_dxi - dt / a.dy * dh_deta;
+            } else if (a.stage == 2) {
+                const double q0 = a.un[k * a.plane + o];
+                q[k] = 3.0 / 4.0 * q0 + 1.0 / 4.0 * q1 - 1.0 / 4.0 * (dt / a.dx * dh_dxi + dt / a.dy * dh_deta);
+            } else {
+                const double q0 = a.un[k * a.plane + o];
+                q[k] = 1.0 / 3.0 * q0 + 2.0 / 3.0 * q1 - 2.0 / 3.0 * (dt / a.dx * dh_dxi + dt / a.dy * dh_deta);
+            }
+        }
+        if (a.if_bodyforce) {
+            // bodyforce = (0, 0, rho, rho v) of the stage input (boundary_rt_general.f90:154-155),
+            // added in its own pass (time_advance_general.f90:71-84)
+            const double coef = a.stage == 1 ? dt : (a.stage == 2 ? 1.0 / 4.0 * dt : TWO_THIRDS_SP * dt);
+            q[2] = q[2] + coef * qs[0];
+            q[3] = q[3] + coef * qs[2];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) a.out[k * a.plane + o] = q[k];
+        double pr[4];
+        prim_of(a.gamma, q, pr);
+        const unsigned long long bi = dbits(fabs(pr[0]) + pr[3]), bj = dbits(fabs(pr[1]) + pr[3]);
+        ei = bi > ei ? bi : ei;
+        ej = bj > ej ? bj : ej;
+    }
+    block_max_to_slot<NT>(ei, ej, st->alpha[(seq + 1) & 3]);
+    launch_epilogue(st, seq, a.last_stage, t, dt, a.dt_hist);
+}
+
+// max|u|+c, max|v|+c of a state (get_primary.f90:69-73), used after a new state is uploaded.
+__global__ void __launch_bounds__(256) max_kernel(StageArgs a)
+{
+    DevState* st = a.st;
+    const int seq = st->seq;
+    unsigned long long ei = 0, ej = 0;
+    const long long n = (long long)a.ni * a.nj;
+    for (long long idx = (long long)blockIdx.x * 256 + threadIdx.x; idx < n; idx += (long long)gridDim.x * 256) {
+        const int gy = (int)(idx / a.ni), gx = (int)(idx - (long long)gy * a.ni);
+        double c[4], pr[4];
+        load_interior(a, a.us, gx, gy, c, pr);
+        const unsigned long long bi = dbits(fabs(pr[0]) + pr[3]), bj = dbits(fabs(pr[1]) + pr[3]);
+        ei = bi > ei ? bi : ei;
+        ej = bj > ej ? bj : ej;
+    }
+    block_max_to_slot<256>(ei, ej, st->alpha[(seq + 1) & 3]);
+    launch_epilogue(st, seq, false, 0.0, 0.0, nullptr);
+}
+
+__global__ void set_clock_kernel(DevState* st, double t, double t_end, int step, int step_end)
+{
+    st->t = t;
+    st->t_end = t_end;
+    st->step = step;
+    st->step_end = step_end;
+    st->hist_n = 0;
+    st->active = (t < t_end) && (step < step_end) && !st->error;
+}
+
+// Padded reference-layout arrays cons(ni+8,nj+8,4), prim(ni+8,nj+8,5) of the
+// current state with ghost layers from the boundary rules at time st->t.
+__global__ void __launch_bounds__(256) fill_padded_kernel(StageArgs a, double* cons, double* prim)
+{
+    const int nip = a.ni + 2 * BF, njp = a.nj + 2 * BF;
+    const long long np = (long long)nip * njp;
+    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= np) return;
+    const int J = (int)(idx / nip), I = (int)(idx - (long long)J * nip);
+    const int gx = I - BF, gy = J - BF;
+    const bool inx = (unsigned)gx < (unsigned)a.ni, iny = (unsigned)gy < (unsigned)a.nj;
+    double c[4] = {0, 0, 0, 0}, pr[4] = {0, 0, 0, 0};
+    double rho = 0.0;
+    if (inx && iny) {
+        load_interior(a, a.us, gx, gy, c, pr);
+        rho = c[0];
+    } else if (inx || iny) {
+        ghost_cell(a, a.us, gx, gy, a.st->t, c, pr);
+        rho = c[0];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) cons[k * np + idx] = c[k];
+    prim[idx] = rho;
+#pragma unroll
+    for (int k = 0; k < 4; k++) prim[(k + 1) * np + idx] = pr[k];
+}
+
+// ------------------------------------------------------------------ launchers
+
+template <int SCHEME, class T>
+static hipError_t launch_one(const StageArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL((stage_kernel<SCHEME, T>), dim3(a.ntiles), dim3(T::NT), 0, s, a);
+    return hipGetLastError();
+}
+
+template <class T>
+static hipError_t launch_scheme(int scheme, const StageArgs& a, hipStream_t s)
+{
+    switch (scheme) {
+    case S_LLF_UP5: return launch_one<S_LLF_UP5, T>(a, s);
+    case S_LLF_JS: return launch_one<S_LLF_JS, T>(a, s);
+    case S_LLF_Z: return launch_one<S_LLF_Z, T>(a, s);
+    case S_GLF_UP5: return launch_one<S_GLF_UP5, T>(a, s);
+    case S_GLF_JS: return launch_one<S_GLF_JS, T>(a, s);
+    case S_GLF_Z: return launch_one<S_GLF_Z, T>(a, s);
+    case S_SW_UP5: return launch_one<S_SW_UP5, T>(a, s);
+    case S_SW_JS: return launch_one<S_SW_JS, T>(a, s);
+    case S_SW_Z: return launch_one<S_SW_Z, T>(a, s);
+    case S_CH: return launch_one<S_CH, T>(a, s);
+    case S_SC: return launch_one<S_SC, T>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+int tile_count(int tile, int ni, int nj, int* tiles_x)
+{
+    int tx, ty, nt;
+    tile_shape(tile, tx, ty, nt);
+    const int nx = (ni + tx - 1) / tx, ny = (nj + ty - 1) / ty;
+    if (tiles_x) *tiles_x = nx;
+    return nx * ny;
+}
+
+hipError_t launch_stage(int scheme, int tile, const StageArgs& a, hipStream_t s)
+{
+    switch (tile) {
+#define X(i, a_, b_, c_) \
+    case i: return launch_scheme<Tile<a_, b_, c_>>(scheme, a, s);
+        DENSE_TILE_LIST(X)
+#undef X
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_max(const StageArgs& a, hipStream_t s)
+{
+    long long n = (long long)a.ni * a.nj;
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(max_kernel, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_clock(DevState* st, double t, double t_end, int step, int step_end, hipStream_t s)
+{
+    hipLaunchKernelGGL(set_clock_kernel, dim3(1), dim3(1), 0, s, st, t, t_end, step, step_end);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_padded(const StageArgs& a, double* cons, double* prim, hipStream_t s)
+{
+    long long np = (long long)(a.ni + 2 * BF) * (a.nj + 2 * BF);
+    hipLaunchKernelGGL(fill_padded_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, a, cons, prim);
+    return hipGetLastError();
+}
+
+}  // namespace dense

@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark: FP64 cell-updates/s of the explicit update on one MI355X.
+
+Workload (one "step" = one full 3-stage SSP-RK3 time step of the hot path):
+double Mach reflection, 1920 x 480 cells on [0,4]x[0,1], Mach 10 shock at
+60 deg, characteristic WENO-Z with global Lax-Friedrichs splitting
+(switch 104), RK3, CFL 0.6, gamma 1.4, FP64.  The state starts from the
+problem's initial condition and stays resident in HBM; the timed region is
+K steps replayed from captured HIP graphs with the clock on the device,
+bracketed by device synchronisation.
+
+  value   = 1920*480*K / wall time of the K steps        (cell-updates/s)
+  roofline: dominant kernel = the fused RK-stage kernel; achieved =
+            algorithmic FP64 flops per launch / average launch duration
+            (HIP events on the library's stream over the timed region);
+            peak = MI355X FP64 vector 78.6 TFLOP/s.
+  cpu_baseline: the reference solver compiled from its sources
+            (oracle/_ref/2dense_ref_racefix: the OpenMP build with the
+            eta-sweep race fixed), run on the host cores on a bounded
+            sample of the same workload, timed by its own CpuTime.dat.
+
+    python bench.py [--gpus 1] [--steps 600] [--warmup 30] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NI, NJ = 1920, 480
+SPLITTING, RECON, CFL = 104, 255, 0.6
+# Algorithmic FP64 ops (+,-,*,/,sqrt; no FMA fusion) per cell-update of this
+# workload, counted by the oracle's op-counting build: python oracle/count_ops.py 5 1920 480 104 255
+FLOPS_PER_CELL_UPDATE = 6063.015625
+# State traffic of a fused ping-pong RK3 step: 4 FP64 variables,
+# read U^n / write U1; read U^n,U1 / write U2; read U^n,U2 / write U^n+1.
+BYTES_PER_CELL_UPDATE = 256.0
+PEAK_FP64_VALU = 78.6e12
+PEAK_HBM = 8.0e12
+
+
+def dmr_initial_state(ni: int, nj: int, gamma: float = 1.4) -> np.ndarray:
+    """Interior conserved state of the double Mach reflection
+    (init_doublemach_general.f90:36-59): post-shock (8, 8.25 cos30, -8.25 sin30, 116.5)
+    left of x = 1/6 + y/sqrt(3), pre-shock (1.4, 0, 0, 1) right of it."""
+    dx, dy = 4.0 / ni, 1.0 / nj
+    x = (np.arange(1, ni + 1, dtype=np.float64) - 0.5) * dx
+    y = (np.arange(1, nj + 1, dtype=np.float64) - 0.5) * dy
+    X, Y = np.meshgrid(x, y)
+    post = X < (1.0 / 6.0 + Y * np.sqrt(1.0 / 3.0))
+    rho = np.where(post, 8.0, 1.4)
+    u = np.where(post, 8.25 * np.sqrt(3.0) * 0.5, 0.0)
+    v = np.where(post, -8.25 * 0.5, 0.0)
+    p = np.where(post, 116.5, 1.0)
+    cons = np.empty((4, nj, ni))
+    cons[0] = rho
+    cons[1] = rho * u
+    cons[2] = rho * v
+    cons[3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
+    return cons
+
+
+def gpu_leg(steps: int, warmup: int):
+    D = importlib.import_module("2dense_amd")
+    cfg = D.make_cfg(NI, NJ, benchmark=5, splitting=SPLITTING, recon=RECON, cfl=CFL, tile=0)
+    s = D.Solver(cfg)
+    s.set_state(dmr_initial_state(NI, NJ))
+    if warmup:
+        s.run(warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    taken = s.run(steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    gpu_ms, launches = s.last_run_gpu_ms()
+    state = s.interior()
+    if not np.all(np.isfinite(state)):
+        raise RuntimeError("non-finite state after the timed run")
+    s.close()
+    return taken, wall, gpu_ms, launches
+
+
+def _threads_available() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_leg(threads: int, steps: int):
+    """Time the reference binary on the host cores (CpuTime.dat timer)."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "2dense_ref_racefix")
+    if os.path.exists(ref):
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        wd = tempfile.mkdtemp(prefix="dense_cpu_")
+        try:
+            po.write_input_in(os.path.join(wd, "input.in"), benchmark=5, ni=NI, nj=NJ, splitting=SPLITTING,
+                              recon=RECON, cfl=CFL, total_step=steps, save_step=10 ** 8, threads=threads,
+                              if_save_binary=False, if_save_tec=False)
+            env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores")
+            subprocess.run([ref], cwd=wd, env=env, capture_output=True, text=True, check=True, timeout=600)
+            with open(os.path.join(wd, "CpuTime.dat")) as f:
+                line = f.readline()
+            cpu_t = float(line.split(":")[1].split()[0].replace("D", "E"))
+            return dict(value=NI * NJ * steps / cpu_t, unit="cell-updates/s", cores=threads, kind="reference",
+                        sample=f"reference solver (flang -O2 -fopenmp, eta-sweep race fixed), DMR {NI}x{NJ} "
+                               f"switch {SPLITTING}, {steps} steps, {threads} OpenMP threads, "
+                               f"timed by its own CpuTime.dat ({cpu_t:.2f} s)")
+        finally:
+            shutil.rmtree(wd, ignore_errors=True)
+    # fallback: the C restatement (bitwise equal to the reference)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    p = po.problem_params(5, NI, NJ, splitting=SPLITTING, recon=RECON, cfl=CFL, nthreads=threads)
+    o = po.Oracle(p)
+    o.init_state(5)
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    return dict(value=NI * NJ * steps / dt, unit="cell-updates/s", cores=threads, kind="port",
+                sample=f"C restatement, DMR {NI}x{NJ} switch {SPLITTING}, {steps} steps, {threads} threads")
+
+
+def read_traffic():
+    """HBM bytes per stage launch from the committed rocprofv3 --pmc summary, if any."""
+    p = os.path.join(ROOT, "profiles", "stage_kernel_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=12)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    taken, wall, gpu_ms, launches = gpu_leg(args.steps, args.warmup)
+    if taken != args.steps:
+        raise RuntimeError(f"ran {taken} of {args.steps} steps")
+    cu = NI * NJ * taken
+    value = cu / wall
+    ms_launch = gpu_ms / max(launches, 1)
+    flops_launch = FLOPS_PER_CELL_UPDATE * NI * NJ / 3.0
+    achieved = flops_launch / (ms_launch * 1e-3)
+    roofline = dict(bound="fp64-valu", achieved=achieved / 1e12, peak=PEAK_FP64_VALU / 1e12, unit="TFLOP/s",
+                    frac=achieved / PEAK_FP64_VALU, traffic=read_traffic(),
+                    ms_per_launch=ms_launch, flops_per_launch=flops_launch,
+                    hbm_algorithmic_gbs=BYTES_PER_CELL_UPDATE / 3.0 * NI * NJ / (ms_launch * 1e-3) / 1e9)
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_leg(_threads_available(), args.cpu_steps)
+        except Exception as e:  # the GPU number stands without the baseline
+            cpu = dict(value=None, error=str(e))
+    out = {
+        "metric": "FP64 cell-updates/s, WENO-Z char. DMR; batched & adjoint vs plain GPU",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": taken,
+        "warmup": args.warmup,
+        "ms_per_step": wall / taken * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (double Mach reflection initial condition)",
+        "config": {"workload": "double Mach reflection 1920x480, characteristic WENO-Z + GLF (switch 104/255), "
+                               "SSP-RK3, CFL 0.6, FP64", "ni": NI, "nj": NJ, "splitting": SPLITTING,
+                   "recon": RECON, "cfl": CFL, "tile": "16x15"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu and cpu.get("value"):
+        out["speedup_vs_cpu"] = value / cpu["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

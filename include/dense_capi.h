@@ -1,0 +1,136 @@
+/*
+ * dense_capi.h -- C ABI of the MI355X explicit-update library (lib2dense_amd.so).
+ *
+ * This is the drop-in boundary for 2DENSE's hot path.  In the reference the
+ * hot path is four parameterless calls per Runge-Kutta stage on module state
+ * (source/utility/general_main.f90:26-31):
+ *     get_primary -> boundary_general -> flux_splitting_general -> time_rk3_general(irk)
+ * A Fortran driver binds the functions below with iso_c_binding (module
+ * 2dense_amd/fortran/dense_capi_mod.f90) and replaces those four calls with
+ * one dense_step() per time step.
+ *
+ * Conventions
+ *  - All arithmetic is IEEE FP64.  Host arrays use the reference layout:
+ *    cons(ni+8, nj+8, 4) and prim(ni+8, nj+8, 6), Fortran column-major,
+ *    i fastest, 4 ghost layers (flow_mod_general.f90:203-205).
+ *  - Every function returns DENSE_OK (0) or a negative DENSE_ERR_* code;
+ *    the library never prints, stops or aborts.  dense_last_error() gives text.
+ *  - A handle is used by one host thread at a time; several handles may coexist.
+ *  - The library owns all device memory; host arrays are owned by the caller.
+ */
+#ifndef DENSE_CAPI_H
+#define DENSE_CAPI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DENSE_OK 0
+#define DENSE_ERR_ARG (-1)         /* bad argument or size */
+#define DENSE_ERR_UNSUPPORTED (-2) /* switch code without an implementation (the reference `stop`s) */
+#define DENSE_ERR_HIP (-3)         /* HIP runtime error */
+#define DENSE_ERR_NONFINITE (-4)   /* non-finite wave speed: the state has blown up */
+#define DENSE_ERR_NOMEM (-5)       /* device allocation failed */
+
+/* Boundary sides and kinds.  benchmark_index 1, 3, 5, 9 select the reference's
+ * own boundary routines (boundary_general.f90:5-31); benchmark 0 takes the
+ * per-side description in dense_cfg.bc / bc_state. */
+enum { DENSE_SIDE_LEFT = 0, DENSE_SIDE_RIGHT = 1, DENSE_SIDE_BOTTOM = 2, DENSE_SIDE_TOP = 3 };
+enum {
+    DENSE_BC_PERIODIC = 0,  /* boundary_ivc_general.f90:13-35, boundary_rt_general.f90:129-151 */
+    DENSE_BC_EXTRAP = 1,    /* boundary_ivc_general.f90:61-102: (4 q1 - q2)/3, cons and prim separately */
+    DENSE_BC_REFLECT = 2,   /* boundary_shockvortex_general.f90:15-79 */
+    DENSE_BC_DIRICHLET = 3, /* boundary_rt_general.f90:14-57, boundary_doublemach_general.f90:98-118 */
+    DENSE_BC_DMR_WALL = 4,  /* boundary_doublemach_general.f90:15-63 (bottom only) */
+    DENSE_BC_DMR_TOP = 5    /* boundary_doublemach_general.f90:65-96 (top only) */
+};
+
+/* Run configuration: the NAMELIST variables the hot path reads
+ * (flow_mod_general.f90:51-180) plus the grid metrics set by init_*. */
+typedef struct dense_cfg {
+    int ni, nj;              /* SINGLE_NI, SINGLE_NJ (interior cells) */
+    int nghost;              /* bfsize_i = bfsize_j; must be 4 */
+    int neq;                 /* NEQ; must be 4 */
+    double dx, dy;           /* block%dx, block%dy */
+    double gamma;            /* GAMMA */
+    double cfl;              /* CFL */
+    double delta;            /* DELTA (AdaWENO threshold) */
+    double dt;               /* DT, used when if_const_dt != 0 */
+    int splitting;           /* SCHEME_SPLITTING_INDEX: 101 LLF, 102 GLF, 103 SW, 104 CH, 105 AdaWENO */
+    int recon;               /* SCHEME_INV_INDEX: 251 upwind5, 253 WENO-JS, 255 WENO-Z */
+    int stages;              /* STAGES: 1, 2 or 3 */
+    int if_const_dt;         /* IF_CONST_DT */
+    int if_bodyforce;        /* gravity source of benchmark 3 (bodyforce = (0,0,rho,rho v)) */
+    int benchmark;           /* BENCHMARK_INDEX: 1, 3, 5, 9, or 0 for bc[] below */
+    int bc[4];               /* per-side kind (benchmark 0) */
+    double bc_state[4][4];   /* per-side Dirichlet (rho,u,v,p); DMR sides: post-shock state */
+    int tile;                /* tile shape index, 0 = default */
+} dense_cfg;
+
+typedef struct dense_solver* dense_handle;
+
+/* Fill cfg->dx, dy, bc, bc_state (and gamma, if_bodyforce for benchmark 3) for
+ * one of the reference's canned problems: init_general.f90:13-37.
+ * Replaces the metric part of init_{IVC,RT,doublemach,shockvortex}_general.f90. */
+int dense_problem_defaults(int benchmark, int ni, int nj, dense_cfg* cfg);
+
+/* Allocate device state for one block.
+ * Replaces allocate_block (flow_mod_general.f90:190-295). */
+int dense_create(const dense_cfg* cfg, dense_handle* out);
+
+/* Free everything (the reference never frees). */
+int dense_destroy(dense_handle h);
+
+/* Upload the conserved state cons(ni+8,nj+8,4) (interior cells are read).
+ * Replaces the module-state hand-off after init_general / load_date_binary
+ * (init_general.f90:13-37, read_WriteAndLoad.f90:30-70). */
+int dense_set_state(dense_handle h, const double* cons);
+
+/* Download cons(ni+8,nj+8,4) and, if prim != NULL, prim(ni+8,nj+8,6).
+ * Interior values are the current state; ghost layers are those the boundary
+ * routines give the current state at the current time.  prim(:,:,6) is the
+ * entropy log(p)-log(rho) on interior cells.  Replaces the get_primary call
+ * before output (general_main.f90:45,60) and the arrays read by
+ * write_tec_at_time / write_date_binary (output_test_general.f90:16-58,
+ * read_WriteAndLoad.f90:1-28). */
+int dense_get_state(dense_handle h, double* cons, double* prim);
+
+/* One time step: `stages` x {get_primary, boundary_general,
+ * flux_splitting_general, time_rk3_general(irk)}
+ * (general_main.f90:26-31).  current_time is t^n, used by time-dependent
+ * boundaries in every stage (boundary_doublemach_general.f90:71).  *dt
+ * receives the step size (CFL or constant, time_advance_general.f90:20-24);
+ * the caller advances its own clock as general_main.f90:35-37 does. */
+int dense_step(dense_handle h, double current_time, double* dt);
+
+/* Up to max_steps steps while (t < total_time && step < total_step)
+ * (general_main.f90:23-24), replayed from a captured HIP graph with the clock
+ * on the device.  On return *t and *step hold the advanced clock; dt_hist,
+ * if not NULL, receives the dt of each step taken (max_steps entries). */
+int dense_run(dense_handle h, int max_steps, double total_time, int total_step,
+              double* t, int* step, double* dt_hist, int* steps_taken);
+
+/* GPU time of the last dense_run, from HIP events recorded on the handle's
+ * stream around the graph launches, and the number of RK-stage kernel
+ * launches that did work in it.  No reference counterpart (the reference
+ * times its loop with OMP_get_wtime, general_main.f90:25-33). */
+int dense_last_run_gpu_ms(dense_handle h, double* ms, int* stage_launches);
+
+/* Global wave speeds max|u|+c, max|v|+c of the current state (get_primary.f90:69-73). */
+int dense_max_eig(dense_handle h, double out[2]);
+
+/* Apply re-read NAMELIST values (gamma, cfl, delta, dt, if_const_dt,
+ * if_bodyforce, splitting, recon) -- general_main.f90:53 re-reads input.in
+ * at every save. */
+int dense_update_params(dense_handle h, const dense_cfg* cfg);
+
+/* Wait for queued device work of this handle. */
+int dense_sync(dense_handle h);
+
+/* Text of the last error on this handle (h may be NULL for create failures). */
+const char* dense_last_error(dense_handle h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,66 @@
+"""Write the golden fixtures: outputs of the reference itself.
+
+Runs the reference solver (compiled from /root/reference/source into
+oracle/_ref/ by oracle/build_ref.sh) through its own input.in and .dat2d
+formats on small grids, for every splitting x reconstruction switch the hot
+path implements, and stores the interior conserved state after N steps plus
+the per-step dt it printed.  These files are data (inputs = the switches in
+`CASES`, outputs = the reference's numbers); run this script in the build
+container, where the reference sources exist.
+
+    python tests/golden/make_golden.py
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", "..", "oracle"))
+import pyoracle as po  # noqa: E402
+
+# (name, benchmark, ni, nj, splitting, recon, cfl, steps, extra namelist)
+CASES = []
+for sp in (101, 102, 103):
+    for rc in (251, 253, 255):
+        CASES.append((f"ivc_{sp}_{rc}", 1, 32, 32, sp, rc, 0.5, 5, {}))
+CASES += [
+    ("ivc_104", 1, 32, 32, 104, 255, 0.5, 5, {}),
+    ("ivc_105", 1, 32, 32, 105, 255, 0.5, 5, {}),
+    ("ivc_101_253_s1", 1, 32, 32, 101, 253, 0.5, 5, {"stages": 1}),
+    ("ivc_101_255_s2", 1, 32, 32, 101, 255, 0.5, 5, {"stages": 2}),
+    ("ivc_101_253_200", 1, 200, 200, 101, 253, 0.5, 3, {}),
+    ("dmr_101_253", 5, 64, 16, 101, 253, 0.6, 10, {}),
+    ("dmr_102_255", 5, 64, 16, 102, 255, 0.6, 10, {}),
+    ("dmr_103_251", 5, 64, 16, 103, 251, 0.6, 10, {}),
+    ("dmr_104", 5, 64, 16, 104, 255, 0.6, 10, {}),
+    ("dmr_105", 5, 64, 16, 105, 255, 0.6, 10, {}),
+    ("dmr_104_120x30_20", 5, 120, 30, 104, 255, 0.6, 20, {}),
+    ("dmr_102_253_constdt", 5, 64, 16, 102, 253, 0.6, 10, {"if_const_dt": True, "dt": 1.0e-4}),
+    ("rt_104", 3, 16, 64, 104, 255, 0.5, 10, {"if_bodyforce": True, "gamma": 1.6666666666666667}),
+    ("rt_105", 3, 16, 64, 105, 255, 0.5, 10, {"if_bodyforce": True, "gamma": 1.6666666666666667}),
+    ("rt_101_255", 3, 16, 64, 101, 255, 0.5, 10, {"if_bodyforce": True, "gamma": 1.6666666666666667}),
+    ("sv_101_255", 9, 64, 32, 101, 255, 0.5, 10, {}),
+    ("sv_104", 9, 64, 32, 104, 255, 0.5, 10, {}),
+    ("sv_105", 9, 64, 32, 105, 255, 0.5, 10, {}),
+]
+
+
+def main():
+    index = {}
+    for name, bm, ni, nj, sp, rc, cfl, steps, extra in CASES:
+        nml = dict(benchmark=bm, ni=ni, nj=nj, splitting=sp, recon=rc, cfl=cfl)
+        nml.update(extra)
+        d = po.run_reference(steps=steps, **nml)
+        cons = d["cons"][:, po.BF:-po.BF, po.BF:-po.BF]
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), cons=cons, dts=d["dts"],
+                            time=np.float64(d["time"]), step=np.int64(d["step"]))
+        index[name] = dict(nml, steps=steps)
+        print(name, cons.shape, "t =", d["time"])
+    with open(os.path.join(HERE, "cases.json"), "w") as f:
+        json.dump(index, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()
