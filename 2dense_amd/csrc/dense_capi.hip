@@ -178,7 +178,7 @@ static int enqueue_step(dense_solver* h)
 
 static void release_graph(dense_solver* h)
 {
-    if (h->graph) hipGraphExecDestroy(h->graph);
+    if (h->graph) (void)hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
     h->graph_steps = 0;
 }
@@ -242,19 +242,19 @@ extern "C" int dense_create(const dense_cfg* cfg, dense_handle* out)
 extern "C" int dense_destroy(dense_handle h)
 {
     if (!h) return DENSE_OK;
-    if (h->stream) hipStreamSynchronize(h->stream);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
     release_graph(h);
     for (int k = 0; k < 3; k++)
-        if (h->U[k]) hipFree(h->U[k]);
-    if (h->st) hipFree(h->st);
-    if (h->st_host) hipHostFree(h->st_host);
-    if (h->dt_hist) hipFree(h->dt_hist);
-    if (h->pad_dev) hipFree(h->pad_dev);
-    if (h->pad_host) hipHostFree(h->pad_host);
-    if (h->stage_host) hipHostFree(h->stage_host);
-    if (h->ev0) hipEventDestroy(h->ev0);
-    if (h->ev1) hipEventDestroy(h->ev1);
-    if (h->stream) hipStreamDestroy(h->stream);
+        if (h->U[k]) (void)hipFree(h->U[k]);
+    if (h->st) (void)hipFree(h->st);
+    if (h->st_host) (void)hipHostFree(h->st_host);
+    if (h->dt_hist) (void)hipFree(h->dt_hist);
+    if (h->pad_dev) (void)hipFree(h->pad_dev);
+    if (h->pad_host) (void)hipHostFree(h->pad_host);
+    if (h->stage_host) (void)hipHostFree(h->stage_host);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return DENSE_OK;
 }
@@ -349,14 +349,14 @@ static int build_graph(dense_solver* h, int nsteps)
     for (int i = 0; i < nsteps; i++) {
         int rc = enqueue_step(h);
         if (rc) {
-            hipStreamEndCapture(h->stream, &g);
-            if (g) hipGraphDestroy(g);
+            (void)hipStreamEndCapture(h->stream, &g);
+            if (g) (void)hipGraphDestroy(g);
             return rc;
         }
     }
     HIPCHK(h, hipStreamEndCapture(h->stream, &g));
     hipError_t e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(h, DENSE_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
     h->graph_steps = nsteps;
     return DENSE_OK;

@@ -153,7 +153,7 @@ __device__ __forceinline__ void load_interior(const StageArgs& a, const double* 
 // deep (k = 1 adjacent).  Reproduces the reference's boundary routines: the
 // ghost conserved state AND the ghost primitive state, which the reference
 // sets independently (e.g. extrapolated separately, boundary_ivc_general.f90:72-80).
-__device__ __noinline__ void ghost_cell(const StageArgs& a, const double* __restrict__ U, int gx, int gy, double t,
+__device__ __forceinline__ void ghost_cell(const StageArgs& a, const double* __restrict__ U, int gx, int gy, double t,
                                         double* c, double* pr)
 {
     const int ni = a.ni, nj = a.nj;

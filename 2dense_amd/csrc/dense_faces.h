@@ -5,6 +5,10 @@
 #pragma once
 #include "dense_device.h"
 
+#ifndef DENSE_SIGN_SERIAL
+#define DENSE_SIGN_SERIAL 0
+#endif
+
 namespace dense {
 
 // LDS record slots of one cell: 0..3 conserved (rho, rho u, rho v, E), then u, v, p, c.
@@ -139,21 +143,28 @@ __device__ __forceinline__ void face_char(const double* __restrict__ s, int base
         Lr[2] = A - B;        Ln[2] = -Cc + D;     Lt[2] = -E;          Le[2] = F;
     }
 
+    // DENSE_SIGN_SERIAL=1 runs the two signs one after the other (fewer live
+    // registers); 0 lets the scheduler interleave them (more ILP per wave).
+#if DENSE_SIGN_SERIAL
+#pragma unroll 1
+#else
 #pragma unroll
+#endif
     for (int sg = 0; sg < 2; sg++) {
         const bool plus = sg == 0;
         const bool ch = plus ? chp : chm;
-        double* hs = h + 4 * sg;  // h[0..3] plus, h[4..7] minus
+        const double sa = plus ? alpha : -alpha;
         // 5-point flux stencil of this sign
         double F[5][4];
 #pragma unroll
         for (int i = 0; i < 5; i++) {
             const int o = base + (plus ? i : 5 - i) * ST;
             const double un = s[VN * NC + o];
-            const double upa = plus ? un + alpha : un - alpha;
+            const double upa = un + sa;
             const double c4[4] = {s[o], s[NC + o], s[2 * NC + o], s[3 * NC + o]};
             lf_flux<DIR>(upa, un, s[REC_P * NC + o], c4, F[i]);
         }
+        double r4[4];
         if (ch) {
             double acc_r = 0, acc_n = 0, acc_t = 0, acc_e = 0;
 #pragma unroll
@@ -196,10 +207,10 @@ __device__ __forceinline__ void face_char(const double* __restrict__ s, int base
                     acc_e = acc_e + (h0 + a0 * un0) * cv;
                 }
             }
-            hs[0] = acc_r;
-            hs[NRM] = acc_n;
-            hs[TAN] = acc_t;
-            hs[3] = acc_e;
+            r4[0] = acc_r;
+            r4[NRM] = acc_n;
+            r4[TAN] = acc_t;
+            r4[3] = acc_e;
         } else {
             const double w0 = plus ? wp0 : wm0, w1 = plus ? wp1 : wm1, w2 = plus ? wp2 : wm2;
 #pragma unroll
@@ -208,8 +219,15 @@ __device__ __forceinline__ void face_char(const double* __restrict__ s, int base
 #pragma unroll
                 for (int i = 0; i < 5; i++) lf[i] = F[i][k];
                 candidates(lf, q0, q1, q2);
-                hs[k] = w0 * q0 + w1 * q1 + w2 * q2;
+                r4[k] = w0 * q0 + w1 * q1 + w2 * q2;
             }
+        }
+        if (plus) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) h[k] = r4[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) h[4 + k] = r4[k];
         }
     }
 #pragma unroll

@@ -42,6 +42,7 @@ struct alignas(64) DevState {
     int seq;                         // kernel sequence number: reads alpha[seq&3], writes alpha[(seq+1)&3]
     int active;                      // 1 while the current step runs
     unsigned int blocks_done;        // last-block detection
+    unsigned int qnext[8];           // per-XCD tile queues of the persistent stage kernel
     int error;                       // 1: non-finite wave speed
     int hist_n;                      // dt history entries written
     int hist_cap;

@@ -96,19 +96,26 @@ __device__ __forceinline__ void block_max_to_slot(unsigned long long ei, unsigne
 // The last workgroup of a launch advances the kernel sequence number, clears
 // the slot the next launch reduces into, checks the new wave speeds and, at
 // the end of a time step, advances the device clock (general_main.f90:35-37).
+// Only atomics cross workgroups here: each block's max has been applied
+// (agent-scope atomic, waited for by the workgroup fence) before its counter
+// increment, so the last block reads final maxima.  Stores to the state
+// arrays and the DevState fields reach the next launch through the kernel
+// boundary, so no device-wide fence (an L2 write-back per block on a
+// multi-XCD part) is needed.
 __device__ void launch_epilogue(DevState* st, int seq, bool end_of_step, double t, double dt, double* dt_hist)
 {
     __syncthreads();
     if (threadIdx.x != 0) return;
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     const unsigned int prev =
-        __hip_atomic_fetch_add(&st->blocks_done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&st->blocks_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (prev != gridDim.x - 1) return;
-    __threadfence();
-    st->blocks_done = 0;
+    __hip_atomic_store(&st->blocks_done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int x = 0; x < 8; x++) __hip_atomic_store(&st->qnext[x], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int out = (seq + 1) & 3, nxt = (seq + 2) & 3;
-    st->alpha[nxt][0] = 0ull;
-    st->alpha[nxt][1] = 0ull;
+    __hip_atomic_store(&st->alpha[nxt][0], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&st->alpha[nxt][1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned long long ai = __hip_atomic_load(&st->alpha[out][0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned long long aj = __hip_atomic_load(&st->alpha[out][1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned long long inf_bits = 0x7ff0000000000000ull;
@@ -121,7 +128,6 @@ __device__ void launch_epilogue(DevState* st, int seq, bool end_of_step, double 
         st->step += 1;
         st->active = (st->t < st->t_end) && (st->step < st->step_end) && !st->error;
     }
-    __threadfence();
 }
 
 // CFL step size, time_advance_general.f90:20-24
@@ -144,11 +150,41 @@ __device__ __forceinline__ void eval_face(const double* __restrict__ s, int base
     else face_cw_lf<DIR, true, SCHEME - S_LLF_UP5, NC, ST>(s, base, alpha, h);
 }
 
+// Minimum waves per SIMD requested from the register allocator, per scheme.
+#ifndef DENSE_WAVES_CHAR
+#define DENSE_WAVES_CHAR 2
+#endif
+#ifndef DENSE_WAVES_CW
+#define DENSE_WAVES_CW 3
+#endif
+template <int SCHEME>
+constexpr int min_waves()
+{
+    return (SCHEME == S_CH || SCHEME == S_SC) ? DENSE_WAVES_CHAR : DENSE_WAVES_CW;
+}
+
+// Per-XCD tile queue of a persistent launch: queue x owns the contiguous
+// tile range [lo, hi); blocks x, x+8, x+16, ... (dealt to XCD x) pull from it.
+__device__ __forceinline__ void queue_range(int x, int n, int& lo, int& hi)
+{
+    const int q = n >> 3, r = n & 7;
+    lo = x * q + (x < r ? x : r);
+    hi = lo + q + (x < r ? 1 : 0);
+}
+
+// Persistent RK-stage kernel: the grid holds only as many workgroups as are
+// resident at once; each loops over tiles pulled from its XCD's queue, so the
+// per-launch work (clock/state reads, the wave-speed reduction and its
+// atomics) is paid once per workgroup rather than once per tile.
 template <int SCHEME, class T>
-__global__ void __launch_bounds__(T::NT) stage_kernel(StageArgs a)
+__global__ void __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(min_waves<SCHEME>())))
+stage_kernel(StageArgs a)
 {
     constexpr int TX = T::TX, TY = T::TY, NT = T::NT, W = T::W, NC = T::NC;
-    __shared__ double s[REC_N * NC];
+    static_assert(TX * TY <= NT, "one update cell per thread");
+    __shared__ double s[REC_N * NC];            // cell records of the tile + halo
+    __shared__ double sh[4 * (T::NXF + T::NYF)];  // h at x faces [4][TY][TX+1], then y faces [4][TY+1][TX]
+    __shared__ int s_tile;
     DevState* st = a.st;
     if (!st->active) return;
     const int seq = st->seq;
@@ -162,88 +198,111 @@ __global__ void __launch_bounds__(T::NT) stage_kernel(StageArgs a)
         dt = st->dt;
     }
     const double t = st->t;
-
-    const int tile = xcd_tile(blockIdx.x, a.ntiles);
-    const int x0 = (tile % a.tiles_x) * TX, y0 = (tile / a.tiles_x) * TY;
     const int ni = a.ni, nj = a.nj;
-
-    // ---- 1. tile + halo -> LDS (corners and cells beyond the 3rd ghost layer are never read)
-    for (int idx = threadIdx.x; idx < NC; idx += NT) {
-        const int ly = idx / W, lx = idx - ly * W;
-        const int gx = x0 + lx - 3, gy = y0 + ly - 3;
-        const bool inx = (unsigned)gx < (unsigned)ni, iny = (unsigned)gy < (unsigned)nj;
-        if ((!inx && !iny) || gx >= ni + 3 || gy >= nj + 3) continue;
-        double c[4], pr[4];
-        if (inx && iny) load_interior(a, a.us, gx, gy, c, pr);
-        else ghost_cell(a, a.us, gx, gy, t, c, pr);
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            s[k * NC + idx] = c[k];
-            s[(4 + k) * NC + idx] = pr[k];
-        }
-    }
-    __syncthreads();
-
-    // ---- 2. faces: x faces (fx in [0,TX], ly in [0,TY)), then y faces
-    double hx[8], hy[8];
     const int tid = threadIdx.x;
-    const int fx = tid % (TX + 1), lyx = tid / (TX + 1);
-    const bool vx = tid < T::NXF && x0 + fx <= ni && y0 + lyx < nj;
-    if (vx) eval_face<SCHEME, 0, NC, 1>(s, (lyx + 3) * W + fx, a, ai, hx);
-    const int lxy = tid % TX, fy = tid / TX;
-    const bool vy = tid < T::NYF && x0 + lxy < ni && y0 + fy <= nj;
-    if (vy) eval_face<SCHEME, 1, NC, W>(s, fy * W + lxy + 3, a, aj, hy);
-    __syncthreads();
-    double* shx = s;                         // [4][TY][TX+1]
-    double* shy = s + 4 * T::NXF;            // [4][TY+1][TX]
-    if (vx) {
-#pragma unroll
-        for (int k = 0; k < 4; k++) shx[k * T::NXF + lyx * (TX + 1) + fx] = hx[k];
-    }
-    if (vy) {
-#pragma unroll
-        for (int k = 0; k < 4; k++) shy[k * T::NYF + fy * TX + lxy] = hy[k];
-    }
-    __syncthreads();
+    const int xq = blockIdx.x & 7;
+    int lo, hi;
+    queue_range(xq, a.ntiles, lo, hi);
+    double* shx = sh;
+    double* shy = sh + 4 * T::NXF;
 
-    // ---- 3. RK stage update, time_advance_general.f90:35-54 / :113-135 / :195-217
+    if (tid == 0) s_tile = lo + (int)atomicAdd(&st->qnext[xq], 1u);
+    __syncthreads();
+    int tile = s_tile;
     unsigned long long ei = 0, ej = 0;
-    for (int idx = tid; idx < TX * TY; idx += NT) {
-        const int lx = idx % TX, ly = idx / TX;
-        const int gx = x0 + lx, gy = y0 + ly;
-        if (gx >= ni || gy >= nj) continue;
-        const long long o = (long long)gy * a.ld + gx;
-        double q[4], qs[4];
+    // per-thread roles in a tile
+    const int fx = tid % (TX + 1), lyx = tid / (TX + 1);
+    const int lxy = tid % TX, fy = tid / TX;
+    const int lxu = tid % TX, lyu = tid / TX;
+
+    while (tile < hi) {
+        const int x0 = (tile % a.tiles_x) * TX, y0 = (tile / a.tiles_x) * TY;
+
+        // ---- 1. tile + halo -> LDS (corners and cells beyond the 3rd ghost layer are never read)
+        for (int idx = tid; idx < NC; idx += NT) {
+            const int ly = idx / W, lx = idx - ly * W;
+            const int gx = x0 + lx - 3, gy = y0 + ly - 3;
+            const bool inx = (unsigned)gx < (unsigned)ni, iny = (unsigned)gy < (unsigned)nj;
+            if ((!inx && !iny) || gx >= ni + 3 || gy >= nj + 3) continue;
+            double c[4], pr[4];
+            if (inx && iny) load_interior(a, a.us, gx, gy, c, pr);
+            else ghost_cell(a, a.us, gx, gy, t, c, pr);
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const double dh_dxi = shx[k * T::NXF + ly * (TX + 1) + lx + 1] - shx[k * T::NXF + ly * (TX + 1) + lx];
-            const double dh_deta = shy[k * T::NYF + (ly + 1) * TX + lx] - shy[k * T::NYF + ly * TX + lx];
-            const double q1 = a.us[k * a.plane + o];
-            qs[k] = q1;
-            if (a.stage == 1) {
-                q[k] = q1 - dt / a.dx * dh_dxi - dt / a.dy * dh_deta;
-            } else if (a.stage == 2) {
-                const double q0 = a.un[k * a.plane + o];
-                q[k] = 3.0 / 4.0 * q0 + 1.0 / 4.0 * q1 - 1.0 / 4.0 * (dt / a.dx * dh_dxi + dt / a.dy * dh_deta);
-            } else {
-                const double q0 = a.un[k * a.plane + o];
-                q[k] = 1.0 / 3.0 * q0 + 2.0 / 3.0 * q1 - 2.0 / 3.0 * (dt / a.dx * dh_dxi + dt / a.dy * dh_deta);
+            for (int k = 0; k < 4; k++) {
+                s[k * NC + idx] = c[k];
+                s[(4 + k) * NC + idx] = pr[k];
             }
         }
-        if (a.if_bodyforce) {
-            // bodyforce = (0, 0, rho, rho v) of the stage input (boundary_rt_general.f90:154-155),
-            // added in its own pass (time_advance_general.f90:71-84)
-            const double coef = a.stage == 1 ? dt : (a.stage == 2 ? 1.0 / 4.0 * dt : TWO_THIRDS_SP * dt);
-            q[2] = q[2] + coef * qs[0];
-            q[3] = q[3] + coef * qs[2];
-        }
+        __syncthreads();
+        if (tid == 0) s_tile = lo + (int)atomicAdd(&st->qnext[xq], 1u);  // next tile, read after the last barrier
+
+        // U^n of this thread's update cell, in flight during the face phase
+        const int gxu = x0 + lxu, gyu = y0 + lyu;
+        const bool vu = tid < TX * TY && gxu < ni && gyu < nj;
+        const long long ou = (long long)gyu * a.ld + gxu;
+        double q0[4] = {0, 0, 0, 0};
+        if (vu && a.stage > 1) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) a.out[k * a.plane + o] = q[k];
-        double pr[4];
-        prim_of(a.gamma, q, pr);
-        const unsigned long long bi = dbits(fabs(pr[0]) + pr[3]), bj = dbits(fabs(pr[1]) + pr[3]);
-        ei = bi > ei ? bi : ei;
-        ej = bj > ej ? bj : ej;
+            for (int k = 0; k < 4; k++) q0[k] = a.un[k * a.plane + ou];
+        }
+
+        // ---- 2. faces: x faces (fx in [0,TX], ly in [0,TY)), then y faces
+        {
+            double hx[8];
+            const bool vx = tid < T::NXF && x0 + fx <= ni && y0 + lyx < nj;
+            if (vx) {
+                eval_face<SCHEME, 0, NC, 1>(s, (lyx + 3) * W + fx, a, ai, hx);
+#pragma unroll
+                for (int k = 0; k < 4; k++) shx[k * T::NXF + lyx * (TX + 1) + fx] = hx[k];
+            }
+        }
+        {
+            double hy[8];
+            const bool vy = tid < T::NYF && x0 + lxy < ni && y0 + fy <= nj;
+            if (vy) {
+                eval_face<SCHEME, 1, NC, W>(s, fy * W + lxy + 3, a, aj, hy);
+#pragma unroll
+                for (int k = 0; k < 4; k++) shy[k * T::NYF + fy * TX + lxy] = hy[k];
+            }
+        }
+        __syncthreads();
+
+        // ---- 3. RK stage update, time_advance_general.f90:35-54 / :113-135 / :195-217
+        if (vu) {
+            const int ic = (lyu + 3) * W + lxu + 3;  // the cell's record: stage input U^(s-1)
+            double q[4], qs[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const double dh_dxi =
+                    shx[k * T::NXF + lyu * (TX + 1) + lxu + 1] - shx[k * T::NXF + lyu * (TX + 1) + lxu];
+                const double dh_deta = shy[k * T::NYF + (lyu + 1) * TX + lxu] - shy[k * T::NYF + lyu * TX + lxu];
+                const double q1 = s[k * NC + ic];
+                qs[k] = q1;
+                if (a.stage == 1) {
+                    q[k] = q1 - dt / a.dx * dh_dxi - dt / a.dy * dh_deta;
+                } else if (a.stage == 2) {
+                    q[k] = 3.0 / 4.0 * q0[k] + 1.0 / 4.0 * q1 - 1.0 / 4.0 * (dt / a.dx * dh_dxi + dt / a.dy * dh_deta);
+                } else {
+                    q[k] = 1.0 / 3.0 * q0[k] + 2.0 / 3.0 * q1 - 2.0 / 3.0 * (dt / a.dx * dh_dxi + dt / a.dy * dh_deta);
+                }
+            }
+            if (a.if_bodyforce) {
+                // bodyforce = (0, 0, rho, rho v) of the stage input (boundary_rt_general.f90:154-155),
+                // added in its own pass (time_advance_general.f90:71-84)
+                const double coef = a.stage == 1 ? dt : (a.stage == 2 ? 1.0 / 4.0 * dt : TWO_THIRDS_SP * dt);
+                q[2] = q[2] + coef * qs[0];
+                q[3] = q[3] + coef * qs[2];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) a.out[k * a.plane + ou] = q[k];
+            double pr[4];
+            prim_of(a.gamma, q, pr);
+            const unsigned long long bi = dbits(fabs(pr[0]) + pr[3]), bj = dbits(fabs(pr[1]) + pr[3]);
+            ei = bi > ei ? bi : ei;
+            ej = bj > ej ? bj : ej;
+        }
+        __syncthreads();  // LDS records and sh are rewritten by the next tile; s_tile is final
+        tile = s_tile;
     }
     block_max_to_slot<NT>(ei, ej, st->alpha[(seq + 1) & 3]);
     launch_epilogue(st, seq, a.last_stage, t, dt, a.dt_hist);
@@ -307,10 +366,26 @@ __global__ void __launch_bounds__(256) fill_padded_kernel(StageArgs a, double* c
 
 // ------------------------------------------------------------------ launchers
 
+// Persistent grid: the resident capacity of the device (blocks per CU from the
+// occupancy calculator times the CU count), a multiple of 8 so that block
+// b sits on XCD b % 8, and no larger than needed for the tiles.
 template <int SCHEME, class T>
 static hipError_t launch_one(const StageArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((stage_kernel<SCHEME, T>), dim3(a.ntiles), dim3(T::NT), 0, s, a);
+    static int capacity = 0;
+    if (capacity == 0) {
+        int dev = 0, per_cu = 0, cus = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, stage_kernel<SCHEME, T>, T::NT, 0);
+        if (e != hipSuccess) return e;
+        capacity = (per_cu < 1 ? 1 : per_cu) * (cus < 1 ? 1 : cus);
+    }
+    int grid = (a.ntiles + 7) / 8 * 8;
+    if (grid > capacity) grid = capacity / 8 * 8;
+    if (grid < 8) grid = 8;
+    hipLaunchKernelGGL((stage_kernel<SCHEME, T>), dim3(grid), dim3(T::NT), 0, s, a);
     return hipGetLastError();
 }
 

@@ -19,7 +19,10 @@ bracketed by device synchronisation.
             eta-sweep race fixed), run on the host cores on a bounded
             sample of the same workload, timed by its own CpuTime.dat.
 
-    python bench.py [--gpus 1] [--steps 600] [--warmup 30] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 640] [--warmup 32] [--no-cpu]
+
+Steps are replayed in graph chunks of 16; multiples of 16 keep every
+recorded stage launch a working one.
 """
 from __future__ import annotations
 
@@ -134,6 +137,19 @@ def cpu_leg(threads: int, steps: int):
                 sample=f"C restatement, DMR {NI}x{NJ} switch {SPLITTING}, {steps} steps, {threads} threads")
 
 
+def copy_probe(nbytes: int):
+    """One plain device copy of nbytes, to calibrate the HBM counters: the
+    FETCH_SIZE/WRITE_SIZE a profiler reports for it should equal nbytes."""
+    import torch
+    n = nbytes // 8
+    src = torch.arange(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 8}))
+
+
 def read_traffic():
     """HBM bytes per stage launch from the committed rocprofv3 --pmc summary, if any."""
     p = os.path.join(ROOT, "profiles", "stage_kernel_traffic.json")
@@ -146,11 +162,16 @@ def read_traffic():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=600)
-    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=640)
+    ap.add_argument("--warmup", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run one plain device-to-device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU sharding is design-only)", file=sys.stderr)
