@@ -53,7 +53,8 @@ static int fail(dense_solver* h, int code, const char* fmt, ...)
         if (e_ != hipSuccess) return fail(h, DENSE_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
-static int validate(const dense_cfg* c, char* why, size_t n)
+namespace dense {
+int validate_cfg(const dense_cfg* c, char* why, size_t n)
 {
     if (c->ni < 4 || c->nj < 4) return snprintf(why, n, "ni, nj must be >= 4 (got %d, %d)", c->ni, c->nj), DENSE_ERR_ARG;
     if ((long long)c->ni * c->nj > (1ll << 31)) return snprintf(why, n, "grid too large"), DENSE_ERR_ARG;
@@ -77,6 +78,7 @@ static int validate(const dense_cfg* c, char* why, size_t n)
     if (need < 4) return snprintf(why, n, "grid must have at least 4 cells per direction"), DENSE_ERR_ARG;
     return DENSE_OK;
 }
+}  // namespace dense
 
 extern "C" int dense_problem_defaults(int benchmark, int ni, int nj, dense_cfg* c)
 {
@@ -201,7 +203,7 @@ extern "C" int dense_create(const dense_cfg* cfg, dense_handle* out)
         if (!(c.dy > 0)) c.dy = d.dy;
     }
     char why[256];
-    int rc = validate(&c, why, sizeof why);
+    int rc = validate_cfg(&c, why, sizeof why);
     if (rc) return fail(nullptr, rc, "%s", why);
     dense_solver* h = new (std::nothrow) dense_solver;
     if (!h) return fail(nullptr, DENSE_ERR_NOMEM, "host allocation failed");
@@ -432,7 +434,7 @@ extern "C" int dense_update_params(dense_handle h, const dense_cfg* cfg)
     c.splitting = cfg->splitting;
     c.recon = cfg->recon;
     char why[256];
-    int rc = validate(&c, why, sizeof why);
+    int rc = validate_cfg(&c, why, sizeof why);
     if (rc) return fail(h, rc, "%s", why);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->cfg = c;

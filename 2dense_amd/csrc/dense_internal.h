@@ -70,6 +70,41 @@ struct StageArgs {
     BcDesc bc;
 };
 
+// ---- batched mode: many independent blocks in one launch per RK stage
+struct BatchInst {          // static description of one instance (device array)
+    double* U[3];           // U^n, stage 1, stage 2 (same layout as a single block)
+    int ni, nj, ld, tiles_x;
+    long long plane;
+    int tile_base, ntiles;  // its tiles in the batch's tile list
+    double dx, dy, gamma, cfl, delta, dt_const;
+    int if_const_dt, if_bodyforce;
+    BcDesc bc;
+};
+struct alignas(64) BatchInstState {  // per-instance clock and reduction slots
+    unsigned long long alpha[4][2];
+    double t, dt;
+    int step, error;
+};
+struct alignas(64) BatchState {      // batch-wide launch bookkeeping
+    int seq, active, step, step_end;
+    unsigned int blocks_done;
+    unsigned int qnext[8];
+};
+struct BatchArgs {
+    const BatchInst* inst;
+    BatchInstState* ist;
+    BatchState* bst;
+    const int* tile_inst;   // global tile -> instance
+    double* dt_hist;        // [step][instance], hist_cap steps
+    int hist_cap;
+    int ninst, ntiles;
+    int stage, last_stage, nstages;
+    double sqrt3;
+};
+
+hipError_t launch_batch_stage(int scheme, int tile, int grid_pct, const BatchArgs& b, hipStream_t s);
+hipError_t launch_batch_max(const BatchArgs& b, hipStream_t s);
+
 // host-side launchers (dense_kernels.hip)
 hipError_t launch_stage(int scheme, int tile, const StageArgs& a, hipStream_t s);
 hipError_t launch_max(const StageArgs& a, hipStream_t s);

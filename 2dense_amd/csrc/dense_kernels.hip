@@ -1,176 +1,8 @@
-// Fused RK-stage kernel of the explicit update and its helpers (gfx950).
-//
-// One workgroup owns a TX x TY tile of interior cells.  Per stage it
-//   1. stages the tile plus a 3-cell halo of conserved variables in LDS with
-//      coalesced loads along i, deriving primitives on the fly
-//      (get_primary.f90:46-67) and synthesising ghost cells from the boundary
-//      rules (boundary_*_general.f90) so no ghost layer lives in HBM;
-//   2. reconstructs all x faces, then all y faces of the tile (one face per
-//      thread), each face producing h = h+ + h- (flux_splitting_*.f90,
-//      scheme_*.f90, field_op.f90:1-21);
-//   3. applies the SSP-RK3 stage update (time_advance_general.f90) and
-//      reduces max|u|+c, max|v|+c of the new state with wavefront shuffles
-//      for the next stage's global Lax-Friedrichs speed and CFL step.
-// Nothing here is an MFMA contraction; the kernel is FP64-VALU bound.
-#include "dense_faces.h"
+// Fused RK-stage kernel of the explicit update (one block) and its helpers
+// (gfx950).  The per-tile work lives in dense_tile.h.
+#include "dense_tile.h"
 
 namespace dense {
-
-template <int TX_, int TY_, int NT_>
-struct Tile {
-    static constexpr int TX = TX_, TY = TY_, NT = NT_;
-    static constexpr int W = TX + 6, H = TY + 6, NC = W * H;
-    static constexpr int NXF = (TX + 1) * TY, NYF = TX * (TY + 1);
-    static_assert(NXF <= NT && NYF <= NT, "one face per thread per direction");
-    static_assert(4 * (NXF + NYF) <= REC_N * NC, "h arrays alias the cell record");
-};
-
-// Tile shapes (index = dense_cfg.tile).  Chosen so that each direction has
-// at most one face per thread and a multiple of 64 threads.
-#define DENSE_TILE_LIST(X) \
-    X(0, 16, 15, 256)      \
-    X(1, 32, 7, 256)       \
-    X(2, 8, 7, 64)         \
-    X(3, 32, 15, 512)
-
-constexpr int NTILES_SHAPES = 4;
-
-__host__ __device__ inline void tile_shape(int tile, int& tx, int& ty, int& nt)
-{
-    switch (tile) {
-#define X(i, a, b, c) \
-    case i: tx = a; ty = b; nt = c; return;
-        DENSE_TILE_LIST(X)
-#undef X
-    default: tx = 16; ty = 15; nt = 256; return;
-    }
-}
-
-// Block -> tile map that hands each XCD a contiguous run of tiles (workgroups
-// are dealt round-robin over the 8 XCDs), so halo re-reads hit the same L2.
-__device__ __forceinline__ int xcd_tile(int b, int n)
-{
-    const int q = n >> 3, r = n & 7;
-    const int x = b & 7, k = b >> 3;
-    return x * q + (x < r ? x : r) + k;
-}
-
-__device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
-
-__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(v, off, 64);
-        v = o > v ? o : v;
-    }
-    return v;
-}
-
-// Block-wide max of (ei, ej) bit patterns into the out slot, skipping the
-// atomic when the slot already holds a larger value.
-template <int NT>
-__device__ __forceinline__ void block_max_to_slot(unsigned long long ei, unsigned long long ej, unsigned long long* slot)
-{
-    __shared__ unsigned long long red[2][NT / 64];
-    ei = wave_max_u64(ei);
-    ej = wave_max_u64(ej);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) {
-        red[0][w] = ei;
-        red[1][w] = ej;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long mi = red[0][0], mj = red[1][0];
-#pragma unroll
-        for (int k = 1; k < NT / 64; k++) {
-            mi = red[0][k] > mi ? red[0][k] : mi;
-            mj = red[1][k] > mj ? red[1][k] : mj;
-        }
-        if (mi > __hip_atomic_load(&slot[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&slot[0], mi);
-        if (mj > __hip_atomic_load(&slot[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(&slot[1], mj);
-    }
-}
-
-// The last workgroup of a launch advances the kernel sequence number, clears
-// the slot the next launch reduces into, checks the new wave speeds and, at
-// the end of a time step, advances the device clock (general_main.f90:35-37).
-// Only atomics cross workgroups here: each block's max has been applied
-// (agent-scope atomic, waited for by the workgroup fence) before its counter
-// increment, so the last block reads final maxima.  Stores to the state
-// arrays and the DevState fields reach the next launch through the kernel
-// boundary, so no device-wide fence (an L2 write-back per block on a
-// multi-XCD part) is needed.
-__device__ void launch_epilogue(DevState* st, int seq, bool end_of_step, double t, double dt, double* dt_hist)
-{
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    const unsigned int prev =
-        __hip_atomic_fetch_add(&st->blocks_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (prev != gridDim.x - 1) return;
-    __hip_atomic_store(&st->blocks_done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#pragma unroll
-    for (int x = 0; x < 8; x++) __hip_atomic_store(&st->qnext[x], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int out = (seq + 1) & 3, nxt = (seq + 2) & 3;
-    __hip_atomic_store(&st->alpha[nxt][0], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&st->alpha[nxt][1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long ai = __hip_atomic_load(&st->alpha[out][0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long aj = __hip_atomic_load(&st->alpha[out][1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long inf_bits = 0x7ff0000000000000ull;
-    if (ai >= inf_bits || aj >= inf_bits) st->error = 1;
-    st->seq = seq + 1;
-    if (end_of_step) {
-        if (dt_hist && st->hist_n < st->hist_cap) dt_hist[st->hist_n] = dt;
-        st->hist_n += 1;
-        st->t = t + dt;
-        st->step += 1;
-        st->active = (st->t < st->t_end) && (st->step < st->step_end) && !st->error;
-    }
-}
-
-// CFL step size, time_advance_general.f90:20-24
-__device__ __forceinline__ double cfl_dt(const StageArgs& a, double ai, double aj)
-{
-    if (a.if_const_dt) return a.dt_const;
-    const double dt_xi = a.dx / ai;
-    const double dt_eta = a.dy / aj;
-    return a.cfl * dt_xi * dt_eta / (dt_xi + dt_eta);
-}
-
-template <int SCHEME, int DIR, int NC, int ST>
-__device__ __forceinline__ void eval_face(const double* __restrict__ s, int base, const StageArgs& a, double alpha,
-                                          double* h)
-{
-    if constexpr (SCHEME == S_CH) face_char<DIR, false, NC, ST>(s, base, alpha, a.gamma, a.delta, h);
-    else if constexpr (SCHEME == S_SC) face_char<DIR, true, NC, ST>(s, base, alpha, a.gamma, a.delta, h);
-    else if constexpr (SCHEME >= S_SW_UP5) face_cw_sw<DIR, SCHEME - S_SW_UP5, NC, ST>(s, base, a.gamma, h);
-    else if constexpr (SCHEME >= S_GLF_UP5) face_cw_lf<DIR, false, SCHEME - S_GLF_UP5, NC, ST>(s, base, alpha, h);
-    else face_cw_lf<DIR, true, SCHEME - S_LLF_UP5, NC, ST>(s, base, alpha, h);
-}
-
-// Minimum waves per SIMD requested from the register allocator, per scheme.
-#ifndef DENSE_WAVES_CHAR
-#define DENSE_WAVES_CHAR 2
-#endif
-#ifndef DENSE_WAVES_CW
-#define DENSE_WAVES_CW 3
-#endif
-template <int SCHEME>
-constexpr int min_waves()
-{
-    return (SCHEME == S_CH || SCHEME == S_SC) ? DENSE_WAVES_CHAR : DENSE_WAVES_CW;
-}
-
-// Per-XCD tile queue of a persistent launch: queue x owns the contiguous
-// tile range [lo, hi); blocks x, x+8, x+16, ... (dealt to XCD x) pull from it.
-__device__ __forceinline__ void queue_range(int x, int n, int& lo, int& hi)
-{
-    const int q = n >> 3, r = n & 7;
-    lo = x * q + (x < r ? x : r);
-    hi = lo + q + (x < r ? 1 : 0);
-}
 
 // Persistent RK-stage kernel: the grid holds only as many workgroups as are
 // resident at once; each loops over tiles pulled from its XCD's queue, so the
@@ -180,10 +12,7 @@ template <int SCHEME, class T>
 __global__ void __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(min_waves<SCHEME>())))
 stage_kernel(StageArgs a)
 {
-    constexpr int TX = T::TX, TY = T::TY, NT = T::NT, W = T::W, NC = T::NC;
-    static_assert(TX * TY <= NT, "one update cell per thread");
-    __shared__ double s[REC_N * NC];            // cell records of the tile + halo
-    __shared__ double sh[4 * (T::NXF + T::NYF)];  // h at x faces [4][TY][TX+1], then y faces [4][TY+1][TX]
+    __shared__ TileLds<T> L;
     __shared__ int s_tile;
     DevState* st = a.st;
     if (!st->active) return;
@@ -198,113 +27,19 @@ stage_kernel(StageArgs a)
         dt = st->dt;
     }
     const double t = st->t;
-    const int ni = a.ni, nj = a.nj;
-    const int tid = threadIdx.x;
     const int xq = blockIdx.x & 7;
     int lo, hi;
     queue_range(xq, a.ntiles, lo, hi);
-    double* shx = sh;
-    double* shy = sh + 4 * T::NXF;
-
-    if (tid == 0) s_tile = lo + (int)atomicAdd(&st->qnext[xq], 1u);
+    if (threadIdx.x == 0) s_tile = lo + (int)atomicAdd(&st->qnext[xq], 1u);
     __syncthreads();
     int tile = s_tile;
     unsigned long long ei = 0, ej = 0;
-    // per-thread roles in a tile
-    const int fx = tid % (TX + 1), lyx = tid / (TX + 1);
-    const int lxy = tid % TX, fy = tid / TX;
-    const int lxu = tid % TX, lyu = tid / TX;
-
     while (tile < hi) {
-        const int x0 = (tile % a.tiles_x) * TX, y0 = (tile / a.tiles_x) * TY;
-
-        // ---- 1. tile + halo -> LDS (corners and cells beyond the 3rd ghost layer are never read)
-        for (int idx = tid; idx < NC; idx += NT) {
-            const int ly = idx / W, lx = idx - ly * W;
-            const int gx = x0 + lx - 3, gy = y0 + ly - 3;
-            const bool inx = (unsigned)gx < (unsigned)ni, iny = (unsigned)gy < (unsigned)nj;
-            if ((!inx && !iny) || gx >= ni + 3 || gy >= nj + 3) continue;
-            double c[4], pr[4];
-            if (inx && iny) load_interior(a, a.us, gx, gy, c, pr);
-            else ghost_cell(a, a.us, gx, gy, t, c, pr);
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                s[k * NC + idx] = c[k];
-                s[(4 + k) * NC + idx] = pr[k];
-            }
-        }
-        __syncthreads();
-        if (tid == 0) s_tile = lo + (int)atomicAdd(&st->qnext[xq], 1u);  // next tile, read after the last barrier
-
-        // U^n of this thread's update cell, in flight during the face phase
-        const int gxu = x0 + lxu, gyu = y0 + lyu;
-        const bool vu = tid < TX * TY && gxu < ni && gyu < nj;
-        const long long ou = (long long)gyu * a.ld + gxu;
-        double q0[4] = {0, 0, 0, 0};
-        if (vu && a.stage > 1) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) q0[k] = a.un[k * a.plane + ou];
-        }
-
-        // ---- 2. faces: x faces (fx in [0,TX], ly in [0,TY)), then y faces
-        {
-            double hx[8];
-            const bool vx = tid < T::NXF && x0 + fx <= ni && y0 + lyx < nj;
-            if (vx) {
-                eval_face<SCHEME, 0, NC, 1>(s, (lyx + 3) * W + fx, a, ai, hx);
-#pragma unroll
-                for (int k = 0; k < 4; k++) shx[k * T::NXF + lyx * (TX + 1) + fx] = hx[k];
-            }
-        }
-        {
-            double hy[8];
-            const bool vy = tid < T::NYF && x0 + lxy < ni && y0 + fy <= nj;
-            if (vy) {
-                eval_face<SCHEME, 1, NC, W>(s, fy * W + lxy + 3, a, aj, hy);
-#pragma unroll
-                for (int k = 0; k < 4; k++) shy[k * T::NYF + fy * TX + lxy] = hy[k];
-            }
-        }
-        __syncthreads();
-
-        // ---- 3. RK stage update, time_advance_general.f90:35-54 / :113-135 / :195-217
-        if (vu) {
-            const int ic = (lyu + 3) * W + lxu + 3;  // the cell's record: stage input U^(s-1)
-            double q[4], qs[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const double dh_dxi =
-                    shx[k * T::NXF + lyu * (TX + 1) + lxu + 1] - shx[k * T::NXF + lyu * (TX + 1) + lxu];
-                const double dh_deta = shy[k * T::NYF + (lyu + 1) * TX + lxu] - shy[k * T::NYF + lyu * TX + lxu];
-                const double q1 = s[k * NC + ic];
-                qs[k] = q1;
-                if (a.stage == 1) {
-                    q[k] = q1 - dt / a.dx * dh_dxi - dt / a.dy * dh_deta;
-                } else if (a.stage == 2) {
-                    q[k] = 3.0 / 4.0 * q0[k] + 1.0 / 4.0 * q1 - 1.0 / 4.0 * (dt / a.dx * dh_dxi + dt / a.dy * dh_deta);
-                } else {
-                    q[k] = 1.0 / 3.0 * q0[k] + 2.0 / 3.0 * q1 - 2.0 / 3.0 * (dt / a.dx * dh_dxi + dt / a.dy * dh_deta);
-                }
-            }
-            if (a.if_bodyforce) {
-                // bodyforce = (0, 0, rho, rho v) of the stage input (boundary_rt_general.f90:154-155),
-                // added in its own pass (time_advance_general.f90:71-84)
-                const double coef = a.stage == 1 ? dt : (a.stage == 2 ? 1.0 / 4.0 * dt : TWO_THIRDS_SP * dt);
-                q[2] = q[2] + coef * qs[0];
-                q[3] = q[3] + coef * qs[2];
-            }
-#pragma unroll
-            for (int k = 0; k < 4; k++) a.out[k * a.plane + ou] = q[k];
-            double pr[4];
-            prim_of(a.gamma, q, pr);
-            const unsigned long long bi = dbits(fabs(pr[0]) + pr[3]), bj = dbits(fabs(pr[1]) + pr[3]);
-            ei = bi > ei ? bi : ei;
-            ej = bj > ej ? bj : ej;
-        }
-        __syncthreads();  // LDS records and sh are rewritten by the next tile; s_tile is final
+        run_tile<SCHEME, T>(a, tile, ai, aj, dt, t, L, ei, ej,
+                            [&] { s_tile = lo + (int)atomicAdd(&st->qnext[xq], 1u); });
         tile = s_tile;
     }
-    block_max_to_slot<NT>(ei, ej, st->alpha[(seq + 1) & 3]);
+    block_max_to_slot<T::NT>(ei, ej, st->alpha[(seq + 1) & 3]);
     launch_epilogue(st, seq, a.last_stage, t, dt, a.dt_hist);
 }
 

@@ -130,6 +130,35 @@ int dense_sync(dense_handle h);
 /* Text of the last error on this handle (h may be NULL for create failures). */
 const char* dense_last_error(dense_handle h);
 
+/* ---------------------------------------------------------------------------
+ * Batched mode (no counterpart in the reference, which advances one block):
+ * n independent blocks of any sizes, same splitting/reconstruction/stages,
+ * each with its own metrics, boundaries, gamma, CFL and clock, advanced by one
+ * persistent launch per RK stage (graph-replayed).  Every instance evolves bit
+ * for bit as it would through dense_step alone.  Needs stages >= 2.
+ * ------------------------------------------------------------------------- */
+typedef struct dense_batch* dense_batch_handle;
+
+int dense_batch_create(const dense_cfg* cfgs, int n, dense_batch_handle* out);
+int dense_batch_destroy(dense_batch_handle b);
+/* Upload instance i's state cons(ni+8,nj+8,4) and its clock t0. */
+int dense_batch_set_state(dense_batch_handle b, int i, const double* cons, double t0);
+/* Download instance i's state (interior; ghost layers zero). */
+int dense_batch_get_state(dense_batch_handle b, int i, double* cons);
+/* nsteps steps of every instance, each with its own CFL dt.  dt_hist (nullable)
+ * receives dt[step*n + i] for the first min(nsteps,256) steps; t_out (nullable)
+ * each instance's clock afterwards. */
+int dense_batch_run(dense_batch_handle b, int nsteps, double* dt_hist, double* t_out);
+/* Tile shape (0..3), instance order (0 as given, 1 largest first) and the
+ * fraction of the resident capacity launched (1..100 %). */
+int dense_batch_set_schedule(dense_batch_handle b, int tile, int order, int grid_pct);
+/* Try every schedule on trial_steps steps of the current states (restored after
+ * each trial) and keep the fastest. */
+int dense_batch_autotune(dense_batch_handle b, int trial_steps, int* tile, int* order, int* grid_pct,
+                         double* best_ms);
+int dense_batch_last_run_gpu_ms(dense_batch_handle b, double* ms);
+const char* dense_batch_last_error(dense_batch_handle b);
+
 #ifdef __cplusplus
 }
 #endif
