@@ -94,6 +94,42 @@ def gpu_leg(steps: int, warmup: int):
     return taken, wall, gpu_ms, launches
 
 
+def batched_leg(n: int = 256, steps: int = 200, seed: int = 2026):
+    """Ensemble of n independent blocks (config: isentropic vortices with
+    strength U[1,5] and shock/vortex interactions with Mach U[1.05,1.5], sides
+    log-uniform 64..512, LLF + WENO-Z, RK3, CFL 0.5, `steps` steps each):
+    sequential plain GPU path (one Solver per instance, graph-replayed steps)
+    vs the batched path (one persistent launch per RK stage for all instances,
+    schedule from the autotuner).  Only the stepping is timed in both."""
+    D = importlib.import_module("2dense_amd")
+    B = importlib.import_module("2dense_amd.batch")
+    cfgs, states, insts = B.make_ensemble(n=n, seed=seed)
+    cells = sum(c.ni * c.nj for c in cfgs)
+    seq_wall = 0.0
+    for cfg, cons in zip(cfgs, states):
+        s = D.Solver(cfg)
+        s.set_state(cons)
+        s.sync()
+        t0 = time.perf_counter()
+        s.run(steps)
+        s.sync()
+        seq_wall += time.perf_counter() - t0
+        s.close()
+    b = B.Batch(cfgs)
+    for i, cons in enumerate(states):
+        b.set_state(i, cons)
+    tile, order, grid_pct, _ = b.autotune(trial_steps=4)
+    t0 = time.perf_counter()
+    b.run(steps)
+    bat_wall = time.perf_counter() - t0
+    b.close()
+    return dict(instances=n, steps=steps, cells=cells, seed=seed,
+                sequential_cell_updates_per_s=cells * steps / seq_wall,
+                batched_cell_updates_per_s=cells * steps / bat_wall,
+                speedup=seq_wall / bat_wall, sequential_s=seq_wall, batched_s=bat_wall,
+                schedule={"tile": tile, "order": order, "grid_pct": grid_pct})
+
+
 def _threads_available() -> int:
     try:
         n = len(os.sched_getaffinity(0))
@@ -166,6 +202,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run one plain device-to-device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -189,6 +226,9 @@ def main():
                     frac=achieved / PEAK_FP64_VALU, traffic=read_traffic(),
                     ms_per_launch=ms_launch, flops_per_launch=flops_launch,
                     hbm_algorithmic_gbs=BYTES_PER_CELL_UPDATE / 3.0 * NI * NJ / (ms_launch * 1e-3) / 1e9)
+    batched = None
+    if not args.no_batched:
+        batched = batched_leg()
     cpu = None
     if not args.no_cpu:
         try:
@@ -213,6 +253,7 @@ def main():
                    "recon": RECON, "cfl": CFL, "tile": "16x15"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     if cpu and cpu.get("value"):
         out["speedup_vs_cpu"] = value / cpu["value"]
