@@ -71,6 +71,14 @@ def load_library(path: str | None = None) -> C.CDLL:
     p = path or os.environ.get("DENSE_LIB", LIB_PATH)
     if not os.path.exists(p):
         raise FileNotFoundError(f"{p} is not built; run `python 2dense_amd/build.py`")
+    # PyTorch-ROCm ships its own HIP runtime under the same soname
+    # (libamdhip64.so.7).  Loaded first, it is the one runtime of the process
+    # and the library binds to it; loaded after the library, torch would bring
+    # up a second runtime on the same device and find no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(p)
     H = C.c_void_p
     dp = C.POINTER(C.c_double)

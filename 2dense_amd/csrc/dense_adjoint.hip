@@ -1,0 +1,556 @@
+// Hand-written FP64 discrete adjoint of the explicit update for
+// Lax-Friedrichs splitting (101) + WENO-Z (255) + SSP-RK3 on a periodic grid.
+//
+// The forward map of one step (time_advance_general.f90:35-217) is
+//   U1 = U0 + dt L(U0);  U2 = 3/4 U0 + 1/4 U1 + 1/4 dt L(U1);
+//   U3 = 1/3 U0 + 2/3 U2 + 2/3 dt L(U2),
+//   L(U)_i = -(h_x(i+1/2) - h_x(i-1/2))/dx - (h_y(j+1/2) - h_y(j-1/2))/dy,
+// with the step sizes frozen (recorded by the forward pass), so the reverse
+// pass is the exact transpose of the discrete map:
+//   g2 = 2/3 g3 + 2/3 dt J(U2)^T g3,  g1 = 1/4 g2 + 1/4 dt J(U1)^T g2,
+//   g0 = g1 + dt J(U0)^T g1 + 1/3 g3 + 3/4 g2.
+// J^T g is applied by two kernels: adj_face_kernel back-propagates
+// lambda_h(face) = c dt (g(f) - g(f-1))/dx through the WENO-Z stencils of both
+// flux signs of every component and writes the 40 stencil adjoints of the
+// face; adj_cell_kernel gathers them per cell in a fixed order (deterministic)
+// and applies the transpose of the Lax-Friedrichs flux Jacobian.
+// Stage states are recomputed segment by segment from checkpoints of U^n taken
+// every `k` steps, with the forward stage kernel (bit-identical to the forward pass).
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "dense_device.h"
+
+namespace dense {
+
+int validate_cfg(const dense_cfg* c, char* why, size_t n);
+
+// WENO-Z value and its reverse-mode derivative: lam_lf[m] += lam_h * dh/dlf[m].
+__device__ __forceinline__ void weno5z_vjp(const double* lf, double lam_h, double* lam_lf)
+{
+    const double q0 = A300 * lf[0] + A301 * lf[1] + A302 * lf[2];
+    const double q1 = A310 * lf[1] + A311 * lf[2] + A312 * lf[3];
+    const double q2 = A320 * lf[2] + A321 * lf[3] + A322 * lf[4];
+    const double X0 = (lf[0] - 2.0 * lf[1]) + lf[2], Y0 = (lf[0] - 4.0 * lf[1]) + 3.0 * lf[2];
+    const double X1 = (lf[1] - 2.0 * lf[2]) + lf[3], Y1 = lf[1] - lf[3];
+    const double X2 = (lf[2] - 2.0 * lf[3]) + lf[4], Y2 = (3.0 * lf[2] - 4.0 * lf[3]) + lf[4];
+    const double b0 = 13.0 * (X0 * X0) / 12.0 + (Y0 * Y0) / 4.0;
+    const double b1 = 13.0 * (X1 * X1) / 12.0 + (Y1 * Y1) / 4.0;
+    const double b2 = 13.0 * (X2 * X2) / 12.0 + (Y2 * Y2) / 4.0;
+    const double d = b2 - b0, tau = fabs(d);
+    const double e0 = WENO_SS + b0, e1 = WENO_SS + b1, e2 = WENO_SS + b2;
+    const double t0 = tau / e0, t1 = tau / e1, t2 = tau / e2;
+    const double a0 = C30 * (1.0 + t0 * t0), a1 = C31 * (1.0 + t1 * t1), a2 = C32 * (1.0 + t2 * t2);
+    const double s = a0 + a1 + a2;
+    const double w0 = a0 / s, w1 = a1 / s, w2 = a2 / s;
+    // reverse
+    const double lq0 = lam_h * w0, lq1 = lam_h * w1, lq2 = lam_h * w2;
+    const double lw0 = lam_h * q0, lw1 = lam_h * q1, lw2 = lam_h * q2;
+    const double dot = lw0 * w0 + lw1 * w1 + lw2 * w2;
+    const double la0 = (lw0 - dot) / s, la1 = (lw1 - dot) / s, la2 = (lw2 - dot) / s;
+    const double lt0 = la0 * C30 * 2.0 * t0, lt1 = la1 * C31 * 2.0 * t1, lt2 = la2 * C32 * 2.0 * t2;
+    const double ltau = lt0 / e0 + lt1 / e1 + lt2 / e2;
+    double lb0 = -lt0 * t0 / e0, lb1 = -lt1 * t1 / e1, lb2 = -lt2 * t2 / e2;
+    const double sg = d > 0.0 ? 1.0 : (d < 0.0 ? -1.0 : 0.0);
+    lb2 += ltau * sg;
+    lb0 -= ltau * sg;
+    const double lX0 = lb0 * (13.0 / 6.0) * X0, lY0 = lb0 * 0.5 * Y0;
+    const double lX1 = lb1 * (13.0 / 6.0) * X1, lY1 = lb1 * 0.5 * Y1;
+    const double lX2 = lb2 * (13.0 / 6.0) * X2, lY2 = lb2 * 0.5 * Y2;
+    lam_lf[0] += lX0 + lY0 + A300 * lq0;
+    lam_lf[1] += -2.0 * lX0 - 4.0 * lY0 + lX1 + lY1 + A301 * lq0 + A310 * lq1;
+    lam_lf[2] += lX0 + 3.0 * lY0 - 2.0 * lX1 + lX2 + 3.0 * lY2 + A302 * lq0 + A311 * lq1 + A320 * lq2;
+    lam_lf[3] += lX1 - lY1 - 2.0 * lX2 - 4.0 * lY2 + A312 * lq1 + A321 * lq2;
+    lam_lf[4] += lX2 + lY2 + A322 * lq2;
+}
+
+// Transpose of the Lax-Friedrichs flux Jacobian of one cell, direction DIR,
+// sign sg (+1/-1): lam_U += (dF/dU)^T lam_F, F = 0.5*((u_n + sg*(|u_n|+c))*U + (0, p e_n, u_n p)).
+template <int DIR>
+__device__ __forceinline__ void lf_flux_vjp(double gamma, const double* U, double sg, const double* lF, double* lU)
+{
+    constexpr int NRM = DIR == 0 ? 1 : 2, TAN = 3 - NRM;
+    const double rho = U[0];
+    const double u = U[1] / rho, v = U[2] / rho;
+    double p = (gamma - 1.0) * (U[3] - 0.5 * rho * (u * u + v * v));
+    const bool floored = p < 0.0;
+    if (floored) p = P_FLOOR;
+    const double c = sqrt(gamma * p / rho);
+    const double un = DIR == 0 ? u : v;
+    const double w = un + sg * (fabs(un) + c);
+    // F0 = .5 w rho, Fn = .5 (w mn + p), Ft = .5 w mt, FE = .5 (w E + un p)
+    const double lw = 0.5 * (lF[0] * U[0] + lF[NRM] * U[NRM] + lF[TAN] * U[TAN] + lF[3] * U[3]);
+    double l[4];
+    l[0] = 0.5 * w * lF[0];
+    l[NRM] = 0.5 * w * lF[NRM];
+    l[TAN] = 0.5 * w * lF[TAN];
+    l[3] = 0.5 * w * lF[3];
+    double lp = 0.5 * lF[NRM] + 0.5 * un * lF[3];
+    double lun = 0.5 * p * lF[3];
+    const double sgn_un = un > 0.0 ? 1.0 : (un < 0.0 ? -1.0 : 0.0);
+    lun += lw * (1.0 + sg * sgn_un);
+    const double lc = sg * lw;
+    // c = sqrt(gamma p / rho)
+    lp += lc * c / (2.0 * p);
+    l[0] += -lc * c / (2.0 * rho);
+    // u_n = m_n / rho
+    l[NRM] += lun / rho;
+    l[0] += -un * lun / rho;
+    // p = (gamma-1)(E - (mx^2 + my^2)/(2 rho))
+    if (!floored) {
+        const double g1 = gamma - 1.0;
+        l[3] += g1 * lp;
+        l[1] += -g1 * u * lp;
+        l[2] += -g1 * v * lp;
+        l[0] += g1 * 0.5 * (u * u + v * v) * lp;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) lU[k] += l[k];
+}
+
+struct AdjArgs {
+    const double* U;   // stage input state, 4 planes of pitch ld
+    const double* g;   // adjoint of the stage output
+    const double* acc_in;
+    double* g_out;
+    double* acc_out;
+    double* lamx;      // [40][nj][ni]: x faces
+    double* lamy;      // [40][nj][ni]: y faces
+    int ni, nj, ld;
+    long long plane;
+    double gamma, cdt_dx, cdt_dy;  // c*dt/dx, c*dt/dy of the stage
+    double bcoef, acoef;           // out = bcoef*g + c dt J^T g (+ acc_in if add_acc); acc_out = acc_in + acoef*g
+    int use_acc_in, add_acc, write_acc;
+};
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+__global__ void __launch_bounds__(256) adj_face_kernel(AdjArgs a)
+{
+    const int ni = a.ni, nj = a.nj;
+    const long long nf = (long long)ni * nj;
+    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= 2 * nf) return;
+    const bool ydir = idx >= nf;
+    const int q = (int)(ydir ? idx - nf : idx);
+    const int r = q / ni, cidx = q - r * ni;  // row r, column cidx
+    // face f between cells f-1 and f along the sweep; cells of the stencil f-3..f+2
+    double lam_h[4];
+    const double scale = ydir ? a.cdt_dy : a.cdt_dx;
+    {
+        const long long o1 = ydir ? (long long)r * a.ld + cidx : (long long)r * a.ld + cidx;
+        const long long o0 = ydir ? (long long)wrapi(r - 1, nj) * a.ld + cidx : (long long)r * a.ld + wrapi(cidx - 1, ni);
+#pragma unroll
+        for (int k = 0; k < 4; k++) lam_h[k] = scale * (a.g[k * a.plane + o1] - a.g[k * a.plane + o0]);
+    }
+    double U[6][4], wp[6], wm[6], pp[6], unp[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        const long long o = ydir ? (long long)wrapi(r - 3 + m, nj) * a.ld + cidx
+                                 : (long long)r * a.ld + wrapi(cidx - 3 + m, ni);
+#pragma unroll
+        for (int k = 0; k < 4; k++) U[m][k] = a.U[k * a.plane + o];
+        double pr[4];
+        prim_of(a.gamma, U[m], pr);
+        const double un = ydir ? pr[1] : pr[0];
+        const double al = fabs(un) + pr[3];
+        wp[m] = un + al;
+        wm[m] = un - al;
+        pp[m] = pr[2];
+        unp[m] = un * pr[2];
+    }
+    const int NRM = ydir ? 2 : 1;
+    double* out = ydir ? a.lamy : a.lamx;
+    const long long fo = (long long)r * ni + cidx;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double lfp[5], lfm[5], lp[5] = {0, 0, 0, 0, 0}, lm[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            const int ip = i, im = 5 - i;
+            const double extp = k == NRM ? pp[ip] : (k == 3 ? unp[ip] : 0.0);
+            const double extm = k == NRM ? pp[im] : (k == 3 ? unp[im] : 0.0);
+            lfp[i] = 0.5 * (wp[ip] * U[ip][k] + extp);
+            lfm[i] = 0.5 * (wm[im] * U[im][k] + extm);
+        }
+        weno5z_vjp(lfp, lam_h[k], lp);
+        weno5z_vjp(lfm, lam_h[k], lm);
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            out[(long long)((0 * 5 + i) * 4 + k) * nf + fo] = lp[i];
+            out[(long long)((1 * 5 + i) * 4 + k) * nf + fo] = lm[i];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) adj_cell_kernel(AdjArgs a)
+{
+    const int ni = a.ni, nj = a.nj;
+    const long long nf = (long long)ni * nj;
+    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= nf) return;
+    const int j = (int)(idx / ni), i = (int)(idx - (long long)j * ni);
+    const long long o = (long long)j * a.ld + i;
+    double U[4], lU[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 4; k++) U[k] = a.U[k * a.plane + o];
+    // x: + stencil of face f covers cells f-3..f+1 (index i - f + 3), - stencil f+2..f-2 (index f + 2 - i)
+    double lF[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double sum = 0.0;
+#pragma unroll
+        for (int m = 0; m < 5; m++) sum += a.lamx[(long long)((0 * 5 + m) * 4 + k) * nf + (long long)j * ni + wrapi(i + 3 - m, ni)];
+        lF[k] = sum;
+    }
+    lf_flux_vjp<0>(a.gamma, U, 1.0, lF, lU);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double sum = 0.0;
+#pragma unroll
+        for (int m = 0; m < 5; m++) sum += a.lamx[(long long)((1 * 5 + m) * 4 + k) * nf + (long long)j * ni + wrapi(i - 2 + m, ni)];
+        lF[k] = sum;
+    }
+    lf_flux_vjp<0>(a.gamma, U, -1.0, lF, lU);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double sum = 0.0;
+#pragma unroll
+        for (int m = 0; m < 5; m++) sum += a.lamy[(long long)((0 * 5 + m) * 4 + k) * nf + (long long)wrapi(j + 3 - m, nj) * ni + i];
+        lF[k] = sum;
+    }
+    lf_flux_vjp<1>(a.gamma, U, 1.0, lF, lU);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double sum = 0.0;
+#pragma unroll
+        for (int m = 0; m < 5; m++) sum += a.lamy[(long long)((1 * 5 + m) * 4 + k) * nf + (long long)wrapi(j - 2 + m, nj) * ni + i];
+        lF[k] = sum;
+    }
+    lf_flux_vjp<1>(a.gamma, U, -1.0, lF, lU);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double g = a.g[k * a.plane + o];
+        const double acc = a.use_acc_in ? a.acc_in[k * a.plane + o] : 0.0;
+        a.g_out[k * a.plane + o] = a.add_acc ? a.bcoef * g + lU[k] + acc : a.bcoef * g + lU[k];
+        if (a.write_acc) a.acc_out[k * a.plane + o] = acc + a.acoef * g;
+    }
+}
+
+// fresh wave-speed ring and queues before a forward or recomputed segment
+__global__ void adj_reset_kernel(DevState* st, int hist_cap)
+{
+    for (int s = 0; s < 4; s++) st->alpha[s][0] = st->alpha[s][1] = 0ull;
+    for (int x = 0; x < 8; x++) st->qnext[x] = 0u;
+    st->seq = 0;
+    st->blocks_done = 0u;
+    st->error = 0;
+    st->active = 0;
+    st->hist_cap = hist_cap;
+}
+
+}  // namespace dense
+
+using namespace dense;
+
+struct dense_adjoint {
+    dense_cfg cfg{};
+    int scheme = -1, nsteps = 0, k = 0, nseg = 0;
+    hipStream_t stream = nullptr;
+    int ld = 0, tiles_x = 0, ntiles = 0;
+    long long plane = 0;
+    DevState* st = nullptr;
+    double* dt_hist = nullptr;
+    double* d_dts = nullptr;
+    std::vector<double> h_dts;
+    double* U = nullptr;     // forward work buffers: 3 states
+    double* ckpt = nullptr;  // nseg states
+    double* seg = nullptr;   // (3k+1) states of the segment being reversed
+    double* gb = nullptr;    // g, g_next, acc (3 states)
+    double* lam = nullptr;   // 80 face planes
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double fwd_ms = 0.0, bwd_ms = 0.0;
+    bool have_forward = false;
+    char err[512] = {0};
+};
+
+static thread_local char g_aerr[512];
+
+static int afail(dense_adjoint* a, int code, const char* fmt, ...)
+{
+    char* buf = a ? a->err : g_aerr;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, 512, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define ACHK(a, call)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) return afail(a, DENSE_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+static double* state_at(double* base, const dense_adjoint* a, long long i) { return base + i * 4 * a->plane; }
+
+static StageArgs adj_stage_args(const dense_adjoint* a)
+{
+    StageArgs s{};
+    const dense_cfg& c = a->cfg;
+    s.st = a->st;
+    s.dt_hist = a->dt_hist;
+    s.ni = c.ni;
+    s.nj = c.nj;
+    s.ld = a->ld;
+    s.plane = a->plane;
+    s.dx = c.dx;
+    s.dy = c.dy;
+    s.gamma = c.gamma;
+    s.cfl = c.cfl;
+    s.delta = c.delta;
+    s.dt_const = c.dt;
+    s.sqrt3 = std::sqrt(3.0);
+    s.if_const_dt = c.if_const_dt;
+    s.tiles_x = a->tiles_x;
+    s.ntiles = a->ntiles;
+    for (int q = 0; q < 4; q++) {
+        s.bc.kind[q] = c.bc[q];
+        for (int v = 0; v < 4; v++) s.bc.state[q][v] = c.bc_state[q][v];
+    }
+    return s;
+}
+
+// one forward step U0 -> (U1, U2) -> U3 with explicit buffers
+static int adj_enqueue_step(dense_adjoint* a, double* U0, double* U1, double* U2, double* U3, const double* sched)
+{
+    StageArgs s = adj_stage_args(a);
+    s.dt_sched = sched;
+    double* outs[3] = {U1, U2, U3};
+    const double* ins[3] = {U0, U1, U2};
+    for (int k = 1; k <= 3; k++) {
+        s.stage = k;
+        s.last_stage = (k == 3);
+        s.un = U0;
+        s.us = ins[k - 1];
+        s.out = outs[k - 1];
+        ACHK(a, launch_stage(a->scheme, a->cfg.tile, s, a->stream));
+    }
+    return DENSE_OK;
+}
+
+// fresh reduction ring + wave speeds of U0 (the stage kernels read them)
+static int adj_prime(dense_adjoint* a, double* U0, double t, int step, int step_end)
+{
+    hipLaunchKernelGGL(adj_reset_kernel, dim3(1), dim3(1), 0, a->stream, a->st, 4096);
+    ACHK(a, hipGetLastError());
+    StageArgs s = adj_stage_args(a);
+    s.us = U0;
+    ACHK(a, launch_max(s, a->stream));
+    ACHK(a, launch_set_clock(a->st, t, INFINITY, step, step_end, a->stream));
+    return DENSE_OK;
+}
+
+extern "C" int dense_adj_create(const dense_cfg* cfg, int nsteps, int checkpoint_every, dense_adj_handle* out)
+{
+    if (!cfg || !out || nsteps < 1 || checkpoint_every < 1) return afail(nullptr, DENSE_ERR_ARG, "bad arguments");
+    *out = nullptr;
+    dense_cfg c = *cfg;
+    char why[256];
+    int rc = validate_cfg(&c, why, sizeof why);
+    if (rc) return afail(nullptr, rc, "%s", why);
+    if (c.splitting != 101 || c.recon != 255 || c.stages != 3)
+        return afail(nullptr, DENSE_ERR_UNSUPPORTED, "the adjoint covers LLF (101) + WENO-Z (255) + RK3");
+    for (int q = 0; q < 4; q++)
+        if (c.bc[q] != DENSE_BC_PERIODIC) return afail(nullptr, DENSE_ERR_UNSUPPORTED, "the adjoint needs periodic boundaries");
+    if (c.if_bodyforce) return afail(nullptr, DENSE_ERR_UNSUPPORTED, "no body force in the adjoint");
+    if (nsteps > 4096) return afail(nullptr, DENSE_ERR_ARG, "at most 4096 steps");
+    dense_adjoint* a = new (std::nothrow) dense_adjoint;
+    if (!a) return afail(nullptr, DENSE_ERR_NOMEM, "host allocation failed");
+    auto bail = [&](int code) {
+        dense_adj_destroy(a);
+        return code;
+    };
+    a->cfg = c;
+    a->scheme = scheme_of(c.splitting, c.recon);
+    a->nsteps = nsteps;
+    a->k = checkpoint_every < nsteps ? checkpoint_every : nsteps;
+    a->nseg = (nsteps + a->k - 1) / a->k;
+    a->ld = (c.ni + 31) / 32 * 32;
+    a->plane = (long long)a->ld * c.nj;
+    a->ntiles = tile_count(c.tile, c.ni, c.nj, &a->tiles_x);
+    const size_t st_bytes = sizeof(double) * 4 * a->plane;
+    const size_t nf = (size_t)c.ni * c.nj;
+    if (hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(afail(nullptr, DENSE_ERR_HIP, "stream creation failed"));
+    for (auto& e : a->ev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(afail(nullptr, DENSE_ERR_HIP, "event creation failed"));
+    if (hipMalloc(&a->st, sizeof(DevState)) != hipSuccess || hipMalloc(&a->dt_hist, sizeof(double) * 4096) != hipSuccess ||
+        hipMalloc(&a->d_dts, sizeof(double) * nsteps) != hipSuccess || hipMalloc(&a->U, 3 * st_bytes) != hipSuccess ||
+        hipMalloc(&a->ckpt, (size_t)a->nseg * st_bytes) != hipSuccess ||
+        hipMalloc(&a->seg, (size_t)(3 * a->k + 1) * st_bytes) != hipSuccess ||
+        hipMalloc(&a->gb, 3 * st_bytes) != hipSuccess || hipMalloc(&a->lam, sizeof(double) * 80 * nf) != hipSuccess)
+        return bail(afail(nullptr, DENSE_ERR_NOMEM, "adjoint allocation failed"));
+    if (hipMemset(a->U, 0, 3 * st_bytes) != hipSuccess || hipMemset(a->seg, 0, (size_t)(3 * a->k + 1) * st_bytes) != hipSuccess ||
+        hipMemset(a->gb, 0, 3 * st_bytes) != hipSuccess)
+        return bail(afail(nullptr, DENSE_ERR_HIP, "memset failed"));
+    a->h_dts.assign(nsteps, 0.0);
+    *out = a;
+    return DENSE_OK;
+}
+
+extern "C" int dense_adj_destroy(dense_adj_handle a)
+{
+    if (!a) return DENSE_OK;
+    if (a->stream) (void)hipStreamSynchronize(a->stream);
+    double* bufs[] = {a->dt_hist, a->d_dts, a->U, a->ckpt, a->seg, a->gb, a->lam};
+    for (double* p : bufs)
+        if (p) (void)hipFree(p);
+    if (a->st) (void)hipFree(a->st);
+    for (auto& e : a->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (a->stream) (void)hipStreamDestroy(a->stream);
+    delete a;
+    return DENSE_OK;
+}
+
+// copy between a packed (4, nj, ni) device array and a pitched state
+static int copy_packed(dense_adjoint* a, double* dst, size_t dpitch, const double* src, size_t spitch)
+{
+    const dense_cfg& c = a->cfg;
+    ACHK(a, hipMemcpy2DAsync(dst, dpitch * sizeof(double), src, spitch * sizeof(double), sizeof(double) * c.ni,
+                             (size_t)4 * c.nj, hipMemcpyDeviceToDevice, a->stream));
+    return DENSE_OK;
+}
+
+extern "C" int dense_adj_forward(dense_adj_handle a, const double* u0_dev, double* un_dev, const double* dt_schedule,
+                                 double* dts_out)
+{
+    if (!a || !u0_dev || !un_dev) return afail(a, DENSE_ERR_ARG, "null argument");
+    const dense_cfg& c = a->cfg;
+    const size_t st_bytes = sizeof(double) * 4 * a->plane;
+    int rc = copy_packed(a, a->U, a->ld, u0_dev, c.ni);
+    if (rc) return rc;
+    if (dt_schedule)
+        ACHK(a, hipMemcpyAsync(a->d_dts, dt_schedule, sizeof(double) * a->nsteps, hipMemcpyHostToDevice, a->stream));
+    if ((rc = adj_prime(a, a->U, 0.0, 0, a->nsteps))) return rc;
+    ACHK(a, hipEventRecord(a->ev[0], a->stream));
+    for (int n = 0; n < a->nsteps; n++) {
+        if (n % a->k == 0)
+            ACHK(a, hipMemcpyAsync(state_at(a->ckpt, a, n / a->k), a->U, st_bytes, hipMemcpyDeviceToDevice, a->stream));
+        // stage 3 writes back in place (it reads U^n only at the cell it writes)
+        if ((rc = adj_enqueue_step(a, a->U, state_at(a->U, a, 1), state_at(a->U, a, 2), a->U,
+                                   dt_schedule ? a->d_dts : nullptr)))
+            return rc;
+    }
+    ACHK(a, hipEventRecord(a->ev[1], a->stream));
+    if (!dt_schedule)
+        ACHK(a, hipMemcpyAsync(a->d_dts, a->dt_hist, sizeof(double) * a->nsteps, hipMemcpyDeviceToDevice, a->stream));
+    if ((rc = copy_packed(a, un_dev, c.ni, a->U, a->ld))) return rc;
+    ACHK(a, hipMemcpyAsync(a->h_dts.data(), a->d_dts, sizeof(double) * a->nsteps, hipMemcpyDeviceToHost, a->stream));
+    DevState sh{};
+    ACHK(a, hipMemcpyAsync(&sh, a->st, sizeof sh, hipMemcpyDeviceToHost, a->stream));
+    ACHK(a, hipStreamSynchronize(a->stream));
+    float ms = 0.f;
+    ACHK(a, hipEventElapsedTime(&ms, a->ev[0], a->ev[1]));
+    a->fwd_ms = ms;
+    if (dts_out) std::memcpy(dts_out, a->h_dts.data(), sizeof(double) * a->nsteps);
+    a->have_forward = true;
+    if (sh.error) return afail(a, DENSE_ERR_NONFINITE, "non-finite wave speed in the forward pass");
+    return DENSE_OK;
+}
+
+static int adj_vjp(dense_adjoint* a, const double* Us, const double* g, double* gout, const double* acc_in,
+                   double* acc_out, bool add_acc, double ccoef, double bcoef, double acoef, double dt)
+{
+    const dense_cfg& c = a->cfg;
+    AdjArgs x{};
+    x.U = Us;
+    x.g = g;
+    x.acc_in = acc_in;
+    x.g_out = gout;
+    x.acc_out = acc_out;
+    x.lamx = a->lam;
+    x.lamy = a->lam + (size_t)40 * c.ni * c.nj;
+    x.ni = c.ni;
+    x.nj = c.nj;
+    x.ld = a->ld;
+    x.plane = a->plane;
+    x.gamma = c.gamma;
+    x.cdt_dx = ccoef * dt / c.dx;
+    x.cdt_dy = ccoef * dt / c.dy;
+    x.bcoef = bcoef;
+    x.acoef = acoef;
+    x.use_acc_in = acc_in != nullptr;
+    x.add_acc = add_acc;
+    x.write_acc = acc_out != nullptr;
+    const long long nf = (long long)c.ni * c.nj;
+    hipLaunchKernelGGL(adj_face_kernel, dim3((unsigned)((2 * nf + 255) / 256)), dim3(256), 0, a->stream, x);
+    ACHK(a, hipGetLastError());
+    hipLaunchKernelGGL(adj_cell_kernel, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, a->stream, x);
+    ACHK(a, hipGetLastError());
+    return DENSE_OK;
+}
+
+extern "C" int dense_adj_backward(dense_adj_handle a, const double* gn_dev, double* g0_dev)
+{
+    if (!a || !gn_dev || !g0_dev) return afail(a, DENSE_ERR_ARG, "null argument");
+    if (!a->have_forward) return afail(a, DENSE_ERR_ARG, "dense_adj_backward before dense_adj_forward");
+    const dense_cfg& c = a->cfg;
+    const size_t st_bytes = sizeof(double) * 4 * a->plane;
+    double* g = state_at(a->gb, a, 0);
+    double* gn = state_at(a->gb, a, 1);
+    double* acc = state_at(a->gb, a, 2);
+    int rc = copy_packed(a, g, a->ld, gn_dev, c.ni);
+    if (rc) return rc;
+    ACHK(a, hipEventRecord(a->ev[2], a->stream));
+    for (int sgi = a->nseg - 1; sgi >= 0; sgi--) {
+        const int n0 = sgi * a->k;
+        const int len = (n0 + a->k <= a->nsteps) ? a->k : a->nsteps - n0;
+        // recompute the segment's stage states from its checkpoint, same kernels and step sizes
+        ACHK(a, hipMemcpyAsync(state_at(a->seg, a, 0), state_at(a->ckpt, a, sgi), st_bytes, hipMemcpyDeviceToDevice,
+                               a->stream));
+        if ((rc = adj_prime(a, state_at(a->seg, a, 0), 0.0, n0, n0 + len))) return rc;
+        for (int j = 0; j < len; j++) {
+            double* U0 = state_at(a->seg, a, 3 * j);
+            if ((rc = adj_enqueue_step(a, U0, state_at(a->seg, a, 3 * j + 1), state_at(a->seg, a, 3 * j + 2),
+                                       state_at(a->seg, a, 3 * j + 3), a->d_dts)))
+                return rc;
+        }
+        // reverse through the segment
+        for (int j = len - 1; j >= 0; j--) {
+            const double dt = a->h_dts[n0 + j];
+            double* U0 = state_at(a->seg, a, 3 * j);
+            double* U1 = state_at(a->seg, a, 3 * j + 1);
+            double* U2 = state_at(a->seg, a, 3 * j + 2);
+            // stage 3: U3 = 1/3 U0 + 2/3 U2 + 2/3 dt L(U2)
+            if ((rc = adj_vjp(a, U2, g, gn, nullptr, acc, false, 2.0 / 3.0, 2.0 / 3.0, 1.0 / 3.0, dt))) return rc;
+            // stage 2: U2 = 3/4 U0 + 1/4 U1 + 1/4 dt L(U1)
+            if ((rc = adj_vjp(a, U1, gn, g, acc, acc, false, 1.0 / 4.0, 1.0 / 4.0, 3.0 / 4.0, dt))) return rc;
+            // stage 1: U1 = U0 + dt L(U0); g0 = g1 + dt J^T g1 + acc
+            if ((rc = adj_vjp(a, U0, g, gn, acc, nullptr, true, 1.0, 1.0, 0.0, dt))) return rc;
+            std::swap(g, gn);
+        }
+    }
+    ACHK(a, hipEventRecord(a->ev[3], a->stream));
+    if ((rc = copy_packed(a, g0_dev, c.ni, g, a->ld))) return rc;
+    ACHK(a, hipStreamSynchronize(a->stream));
+    float ms = 0.f;
+    ACHK(a, hipEventElapsedTime(&ms, a->ev[2], a->ev[3]));
+    a->bwd_ms = ms;
+    return DENSE_OK;
+}
+
+extern "C" int dense_adj_last_ms(dense_adj_handle a, double* fwd_ms, double* bwd_ms)
+{
+    if (!a) return afail(a, DENSE_ERR_ARG, "null handle");
+    if (fwd_ms) *fwd_ms = a->fwd_ms;
+    if (bwd_ms) *bwd_ms = a->bwd_ms;
+    return DENSE_OK;
+}
+
+extern "C" const char* dense_adj_last_error(dense_adj_handle a) { return a ? a->err : g_aerr; }

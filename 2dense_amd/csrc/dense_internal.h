@@ -64,6 +64,7 @@ struct StageArgs {
     long long plane;    // ld * nj
     double dx, dy, gamma, cfl, delta, dt_const, sqrt3;
     int if_const_dt, if_bodyforce;
+    const double* dt_sched;  // if set: dt of step k is dt_sched[k] (frozen schedule)
     int stage;          // 1..3
     int last_stage;
     int tiles_x, ntiles;

@@ -21,7 +21,7 @@ stage_kernel(StageArgs a)
     const double aj = __longlong_as_double((long long)st->alpha[seq & 3][1]);
     double dt;
     if (a.stage == 1) {
-        dt = cfl_dt(a, ai, aj);
+        dt = a.dt_sched ? a.dt_sched[st->step] : cfl_dt(a, ai, aj);
         if (blockIdx.x == 0 && threadIdx.x == 0) st->dt = dt;
     } else {
         dt = st->dt;

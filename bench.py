@@ -19,7 +19,14 @@ bracketed by device synchronisation.
             eta-sweep race fixed), run on the host cores on a bounded
             sample of the same workload, timed by its own CpuTime.dat.
 
-    python bench.py [--gpus 1] [--steps 640] [--warmup 32] [--no-cpu]
+  batched: 256 seeded vortex and shock/vortex blocks (64^2..512^2, LLF +
+            WENO-Z, 200 steps each): sequential plain GPU path vs one
+            persistent launch per RK stage for the whole ensemble.
+  adjoint: gradient of a density mismatch after 200 steps (256^2 periodic
+            vortex, 64 Fourier modes, LLF + WENO-Z, checkpoint every 20 steps)
+            vs a plain forward solve of the same problem.
+
+    python bench.py [--gpus 1] [--steps 640] [--warmup 32] [--no-cpu] [--no-batched] [--no-adjoint]
 
 Steps are replayed in graph chunks of 16; multiples of 16 keep every
 recorded stage launch a working one.
@@ -130,6 +137,77 @@ def batched_leg(n: int = 256, steps: int = 200, seed: int = 2026):
                 schedule={"tile": tile, "order": order, "grid_pct": grid_pct})
 
 
+def adjoint_leg(n: int = 256, steps: int = 200, k: int = 20, n_modes: int = 64, fd_modes: int = 8,
+                reps: int = 3):
+    """Gradient of J = sum (rho_N - rho*)^2 dx dy with respect to 64 Fourier
+    amplitudes of the initial density (periodic 256x256 vortex, LLF + WENO-Z,
+    RK3, CFL 0.4, 200 steps, checkpoint every 20 steps) through the
+    torch.autograd.Function, against a plain forward solve of the same problem
+    (graph-replayed Solver.run).  Reports the cost ratio and a central
+    finite-difference spot check (h = 1e-6) on the first fd_modes amplitudes."""
+    import torch
+    D = importlib.import_module("2dense_amd")
+    A = importlib.import_module("2dense_amd.adjoint")
+    ic = A.FourierIC(n, n_modes=n_modes, device="cuda")
+    theta = torch.tensor(A.draw_amplitudes(n_modes), dtype=torch.float64, device="cuda")
+    target = ic(torch.tensor(A.draw_amplitudes(n_modes, seed=12), dtype=torch.float64, device="cuda"))[0]
+    # plain forward solve
+    cfg = D.make_cfg(n, n, benchmark=0, splitting=D.LLF, recon=D.WENOZ, cfl=0.4, dx=ic.h, dy=ic.h,
+                     bc=[D.BC_PERIODIC] * 4)
+    U0 = ic(theta).cpu().numpy()
+    plain_wall, plain_gpu = [], []
+    with D.Solver(cfg) as s:
+        for _ in range(reps + 1):
+            s.set_state(U0)
+            s.current_time, s.current_step = 0.0, 0
+            s.sync()
+            t0 = time.perf_counter()
+            s.run(steps)
+            s.sync()
+            plain_wall.append(time.perf_counter() - t0)
+            plain_gpu.append(s.last_run_gpu_ms()[0])
+    # gradient: forward with checkpoints + backward, through autograd
+    R = A.Rollout(n, n, steps, dx=ic.h, dy=ic.h, cfl=0.4, checkpoint_every=k)
+    grad_wall, fwd_gpu, bwd_gpu = [], [], []
+    g = None
+    for _ in range(reps + 1):
+        th = theta.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        J = A.density_mismatch(A.rollout(ic(th), R), target, ic.h, ic.h)
+        J.backward()
+        torch.cuda.synchronize()
+        grad_wall.append(time.perf_counter() - t0)
+        f, b = R.last_ms()
+        fwd_gpu.append(f)
+        bwd_gpu.append(b)
+        g = th.grad.cpu().numpy()
+    dts = R.dts.copy()
+
+    def Jat(t):
+        uN, _ = R.forward(ic(t), dt_schedule=dts)
+        return float(A.density_mismatch(uN, target, ic.h, ic.h))
+
+    fd = np.zeros(fd_modes)
+    h = 1e-6
+    for m in range(fd_modes):
+        e = torch.zeros_like(theta)
+        e[m] = h
+        fd[m] = (Jat(theta + e) - Jat(theta - e)) / (2 * h)
+    R.close()
+    med = lambda x: float(np.median(x[1:]))
+    return dict(workload=f"periodic vortex {n}x{n}, {n_modes} Fourier density modes, LLF + WENO-Z (101/255), "
+                         f"SSP-RK3, CFL 0.4, {steps} steps, checkpoint every {k}",
+                plain_forward_ms=med(plain_wall) * 1e3, plain_forward_gpu_ms=med(plain_gpu),
+                gradient_ms=med(grad_wall) * 1e3, adjoint_forward_gpu_ms=med(fwd_gpu),
+                adjoint_backward_gpu_ms=med(bwd_gpu),
+                cost_ratio=med(grad_wall) / med(plain_wall),
+                cost_ratio_gpu=(med(fwd_gpu) + med(bwd_gpu)) / med(plain_gpu),
+                fd_check=dict(modes=fd_modes, h=h,
+                              l2_rel=float(np.linalg.norm(g[:fd_modes] - fd) / np.linalg.norm(fd)),
+                              worst_rel=float(np.max(np.abs(g[:fd_modes] - fd) / np.abs(fd)))))
+
+
 def _threads_available() -> int:
     try:
         n = len(os.sched_getaffinity(0))
@@ -203,6 +281,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run one plain device-to-device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -229,6 +308,9 @@ def main():
     batched = None
     if not args.no_batched:
         batched = batched_leg()
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_leg()
     cpu = None
     if not args.no_cpu:
         try:
@@ -254,6 +336,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "adjoint": adjoint,
     }
     if cpu and cpu.get("value"):
         out["speedup_vs_cpu"] = value / cpu["value"]

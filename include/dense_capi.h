@@ -159,6 +159,34 @@ int dense_batch_autotune(dense_batch_handle b, int trial_steps, int* tile, int* 
 int dense_batch_last_run_gpu_ms(dense_batch_handle b, double* ms);
 const char* dense_batch_last_error(dense_batch_handle b);
 
+/* ---------------------------------------------------------------------------
+ * Discrete adjoint (no counterpart in the reference, which has no gradients):
+ * the exact transpose of nsteps steps of Lax-Friedrichs splitting (101) +
+ * WENO-Z (255) + SSP-RK3 (time_advance_general.f90:35-217) on a periodic
+ * block, with the step sizes frozen.  The forward pass keeps checkpoints of
+ * U^n every checkpoint_every steps; the backward pass recomputes each
+ * segment's stage states from its checkpoint with the forward kernels and
+ * applies hand-written vector-Jacobian products.
+ * States cross this interface as DEVICE pointers to packed float64 arrays
+ * (4, nj, ni) (component, row, column; C order), so a caller that holds its
+ * data in device memory (a torch tensor) never round-trips through the host.
+ * ------------------------------------------------------------------------- */
+typedef struct dense_adjoint* dense_adj_handle;
+
+int dense_adj_create(const dense_cfg* cfg, int nsteps, int checkpoint_every, dense_adj_handle* out);
+int dense_adj_destroy(dense_adj_handle a);
+/* u0_dev -> un_dev over nsteps steps.  dt_schedule (host, nullable): frozen
+ * step sizes; NULL takes the CFL step of every step (time_advance_general.f90:20-24).
+ * dts_out (host, nullable) receives the nsteps step sizes used; the backward
+ * pass uses the same ones. */
+int dense_adj_forward(dense_adj_handle a, const double* u0_dev, double* un_dev, const double* dt_schedule,
+                      double* dts_out);
+/* g0_dev = (d un / d u0)^T gn_dev at the last forward pass. */
+int dense_adj_backward(dense_adj_handle a, const double* gn_dev, double* g0_dev);
+/* GPU time (HIP events) of the last forward and backward passes. */
+int dense_adj_last_ms(dense_adj_handle a, double* fwd_ms, double* bwd_ms);
+const char* dense_adj_last_error(dense_adj_handle a);
+
 #ifdef __cplusplus
 }
 #endif
