@@ -29,7 +29,9 @@ namespace dense {
 
 int validate_cfg(const dense_cfg* c, char* why, size_t n);
 
-// WENO-Z value and its reverse-mode derivative: lam_lf[m] += lam_h * dh/dlf[m].
+// Reverse-mode derivative of WENO-Z: lam_lf[m] += lam_h * dh/dlf[m].  The
+// gradient need not reproduce the forward rounding, so quotients by a shared
+// divisor go through one reciprocal.
 __device__ __forceinline__ void weno5z_vjp(const double* lf, double lam_h, double* lam_lf)
 {
     const double q0 = A300 * lf[0] + A301 * lf[1] + A302 * lf[2];
@@ -38,23 +40,24 @@ __device__ __forceinline__ void weno5z_vjp(const double* lf, double lam_h, doubl
     const double X0 = (lf[0] - 2.0 * lf[1]) + lf[2], Y0 = (lf[0] - 4.0 * lf[1]) + 3.0 * lf[2];
     const double X1 = (lf[1] - 2.0 * lf[2]) + lf[3], Y1 = lf[1] - lf[3];
     const double X2 = (lf[2] - 2.0 * lf[3]) + lf[4], Y2 = (3.0 * lf[2] - 4.0 * lf[3]) + lf[4];
-    const double b0 = 13.0 * (X0 * X0) / 12.0 + (Y0 * Y0) / 4.0;
-    const double b1 = 13.0 * (X1 * X1) / 12.0 + (Y1 * Y1) / 4.0;
-    const double b2 = 13.0 * (X2 * X2) / 12.0 + (Y2 * Y2) / 4.0;
+    constexpr double K13_12 = 13.0 / 12.0;
+    const double b0 = K13_12 * (X0 * X0) + 0.25 * (Y0 * Y0);
+    const double b1 = K13_12 * (X1 * X1) + 0.25 * (Y1 * Y1);
+    const double b2 = K13_12 * (X2 * X2) + 0.25 * (Y2 * Y2);
     const double d = b2 - b0, tau = fabs(d);
-    const double e0 = WENO_SS + b0, e1 = WENO_SS + b1, e2 = WENO_SS + b2;
-    const double t0 = tau / e0, t1 = tau / e1, t2 = tau / e2;
+    const double ie0 = 1.0 / (WENO_SS + b0), ie1 = 1.0 / (WENO_SS + b1), ie2 = 1.0 / (WENO_SS + b2);
+    const double t0 = tau * ie0, t1 = tau * ie1, t2 = tau * ie2;
     const double a0 = C30 * (1.0 + t0 * t0), a1 = C31 * (1.0 + t1 * t1), a2 = C32 * (1.0 + t2 * t2);
-    const double s = a0 + a1 + a2;
-    const double w0 = a0 / s, w1 = a1 / s, w2 = a2 / s;
+    const double is = 1.0 / (a0 + a1 + a2);
+    const double w0 = a0 * is, w1 = a1 * is, w2 = a2 * is;
     // reverse
     const double lq0 = lam_h * w0, lq1 = lam_h * w1, lq2 = lam_h * w2;
     const double lw0 = lam_h * q0, lw1 = lam_h * q1, lw2 = lam_h * q2;
     const double dot = lw0 * w0 + lw1 * w1 + lw2 * w2;
-    const double la0 = (lw0 - dot) / s, la1 = (lw1 - dot) / s, la2 = (lw2 - dot) / s;
-    const double lt0 = la0 * C30 * 2.0 * t0, lt1 = la1 * C31 * 2.0 * t1, lt2 = la2 * C32 * 2.0 * t2;
-    const double ltau = lt0 / e0 + lt1 / e1 + lt2 / e2;
-    double lb0 = -lt0 * t0 / e0, lb1 = -lt1 * t1 / e1, lb2 = -lt2 * t2 / e2;
+    const double la0 = (lw0 - dot) * is, la1 = (lw1 - dot) * is, la2 = (lw2 - dot) * is;
+    const double lt0 = la0 * (2.0 * C30) * t0, lt1 = la1 * (2.0 * C31) * t1, lt2 = la2 * (2.0 * C32) * t2;
+    const double ltau = lt0 * ie0 + lt1 * ie1 + lt2 * ie2;
+    double lb0 = -lt0 * t0 * ie0, lb1 = -lt1 * t1 * ie1, lb2 = -lt2 * t2 * ie2;
     const double sg = d > 0.0 ? 1.0 : (d < 0.0 ? -1.0 : 0.0);
     lb2 += ltau * sg;
     lb0 -= ltau * sg;
@@ -68,20 +71,35 @@ __device__ __forceinline__ void weno5z_vjp(const double* lf, double lam_h, doubl
     lam_lf[4] += lX2 + lY2 + A322 * lq2;
 }
 
+// Primitive variables of one cell for the derivative kernels (reciprocal of rho
+// shared; the forward rounding need not be reproduced).
+struct CellPrim {
+    double ir, u, v, p, c;
+    bool floored;  // pressure floor active: p does not depend on U
+};
+
+__device__ __forceinline__ CellPrim cell_prim(double gamma, const double* U)
+{
+    CellPrim q;
+    q.ir = 1.0 / U[0];
+    q.u = U[1] * q.ir;
+    q.v = U[2] * q.ir;
+    double p = (gamma - 1.0) * (U[3] - 0.5 * U[0] * (q.u * q.u + q.v * q.v));
+    q.floored = p < 0.0;
+    q.p = q.floored ? P_FLOOR : p;
+    q.c = sqrt(gamma * q.p * q.ir);
+    return q;
+}
+
 // Transpose of the Lax-Friedrichs flux Jacobian of one cell, direction DIR,
 // sign sg (+1/-1): lam_U += (dF/dU)^T lam_F, F = 0.5*((u_n + sg*(|u_n|+c))*U + (0, p e_n, u_n p)).
 template <int DIR>
-__device__ __forceinline__ void lf_flux_vjp(double gamma, const double* U, double sg, const double* lF, double* lU)
+__device__ __forceinline__ void lf_flux_vjp(double gamma, const double* U, const CellPrim& q, double sg,
+                                            const double* lF, double* lU)
 {
     constexpr int NRM = DIR == 0 ? 1 : 2, TAN = 3 - NRM;
-    const double rho = U[0];
-    const double u = U[1] / rho, v = U[2] / rho;
-    double p = (gamma - 1.0) * (U[3] - 0.5 * rho * (u * u + v * v));
-    const bool floored = p < 0.0;
-    if (floored) p = P_FLOOR;
-    const double c = sqrt(gamma * p / rho);
-    const double un = DIR == 0 ? u : v;
-    const double w = un + sg * (fabs(un) + c);
+    const double un = DIR == 0 ? q.u : q.v;
+    const double w = un + sg * (fabs(un) + q.c);
     // F0 = .5 w rho, Fn = .5 (w mn + p), Ft = .5 w mt, FE = .5 (w E + un p)
     const double lw = 0.5 * (lF[0] * U[0] + lF[NRM] * U[NRM] + lF[TAN] * U[TAN] + lF[3] * U[3]);
     double l[4];
@@ -90,23 +108,24 @@ __device__ __forceinline__ void lf_flux_vjp(double gamma, const double* U, doubl
     l[TAN] = 0.5 * w * lF[TAN];
     l[3] = 0.5 * w * lF[3];
     double lp = 0.5 * lF[NRM] + 0.5 * un * lF[3];
-    double lun = 0.5 * p * lF[3];
+    double lun = 0.5 * q.p * lF[3];
     const double sgn_un = un > 0.0 ? 1.0 : (un < 0.0 ? -1.0 : 0.0);
     lun += lw * (1.0 + sg * sgn_un);
     const double lc = sg * lw;
-    // c = sqrt(gamma p / rho)
-    lp += lc * c / (2.0 * p);
-    l[0] += -lc * c / (2.0 * rho);
+    // c = sqrt(gamma p / rho): dc/dp = c/(2p), dc/drho = -c/(2 rho)
+    const double hc = 0.5 * lc * q.c;
+    lp += hc / q.p;
+    l[0] -= hc * q.ir;
     // u_n = m_n / rho
-    l[NRM] += lun / rho;
-    l[0] += -un * lun / rho;
+    l[NRM] += lun * q.ir;
+    l[0] -= un * lun * q.ir;
     // p = (gamma-1)(E - (mx^2 + my^2)/(2 rho))
-    if (!floored) {
+    if (!q.floored) {
         const double g1 = gamma - 1.0;
         l[3] += g1 * lp;
-        l[1] += -g1 * u * lp;
-        l[2] += -g1 * v * lp;
-        l[0] += g1 * 0.5 * (u * u + v * v) * lp;
+        l[1] -= g1 * q.u * lp;
+        l[2] -= g1 * q.v * lp;
+        l[0] += g1 * 0.5 * (q.u * q.u + q.v * q.v) * lp;
     }
 #pragma unroll
     for (int k = 0; k < 4; k++) lU[k] += l[k];
@@ -129,116 +148,112 @@ struct AdjArgs {
 
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// One thread per (face, direction, flux sign); blockIdx.y = 2*dir + sign.
+// lambda_h(f) = c dt (g(f) - g(f-1)) / dx is back-propagated through the
+// WENO-Z stencil of that sign for each component; the 5 stencil adjoints of
+// component k go to plane (sign*5 + m)*4 + k of the direction's face array.
+// Stencils: + side lf(m) = F+(f-3+m), - side lf(m) = F-(f+2-m).
 __global__ void __launch_bounds__(256) adj_face_kernel(AdjArgs a)
 {
     const int ni = a.ni, nj = a.nj;
     const long long nf = (long long)ni * nj;
     const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= 2 * nf) return;
-    const bool ydir = idx >= nf;
-    const int q = (int)(ydir ? idx - nf : idx);
-    const int r = q / ni, cidx = q - r * ni;  // row r, column cidx
-    // face f between cells f-1 and f along the sweep; cells of the stencil f-3..f+2
+    if (idx >= nf) return;
+    const bool ydir = blockIdx.y >= 2;
+    const int sign = blockIdx.y & 1;  // 0: F+, 1: F-
+    const int q = (int)idx;
+    const int r = q / ni, cidx = q - r * ni;  // row r, column cidx; face between cells f-1 and f
     double lam_h[4];
     const double scale = ydir ? a.cdt_dy : a.cdt_dx;
     {
-        const long long o1 = ydir ? (long long)r * a.ld + cidx : (long long)r * a.ld + cidx;
+        const long long o1 = (long long)r * a.ld + cidx;
         const long long o0 = ydir ? (long long)wrapi(r - 1, nj) * a.ld + cidx : (long long)r * a.ld + wrapi(cidx - 1, ni);
 #pragma unroll
         for (int k = 0; k < 4; k++) lam_h[k] = scale * (a.g[k * a.plane + o1] - a.g[k * a.plane + o0]);
     }
-    double U[6][4], wp[6], wm[6], pp[6], unp[6];
+    const double sg = sign ? -1.0 : 1.0;
+    double U[5][4], w[5], pp[5], unp[5];
 #pragma unroll
-    for (int m = 0; m < 6; m++) {
-        const long long o = ydir ? (long long)wrapi(r - 3 + m, nj) * a.ld + cidx
-                                 : (long long)r * a.ld + wrapi(cidx - 3 + m, ni);
+    for (int m = 0; m < 5; m++) {
+        const int off = sign ? 2 - m : m - 3;  // stencil cell of lf(m), relative to f
+        const long long o = ydir ? (long long)wrapi(r + off, nj) * a.ld + cidx
+                                 : (long long)r * a.ld + wrapi(cidx + off, ni);
 #pragma unroll
         for (int k = 0; k < 4; k++) U[m][k] = a.U[k * a.plane + o];
-        double pr[4];
-        prim_of(a.gamma, U[m], pr);
-        const double un = ydir ? pr[1] : pr[0];
-        const double al = fabs(un) + pr[3];
-        wp[m] = un + al;
-        wm[m] = un - al;
-        pp[m] = pr[2];
-        unp[m] = un * pr[2];
+        const CellPrim cp = cell_prim(a.gamma, U[m]);
+        const double un = ydir ? cp.v : cp.u;
+        w[m] = un + sg * (fabs(un) + cp.c);
+        pp[m] = cp.p;
+        unp[m] = un * cp.p;
     }
     const int NRM = ydir ? 2 : 1;
     double* out = ydir ? a.lamy : a.lamx;
     const long long fo = (long long)r * ni + cidx;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        double lfp[5], lfm[5], lp[5] = {0, 0, 0, 0, 0}, lm[5] = {0, 0, 0, 0, 0};
+        double lf[5], ll[5] = {0, 0, 0, 0, 0};
 #pragma unroll
-        for (int i = 0; i < 5; i++) {
-            const int ip = i, im = 5 - i;
-            const double extp = k == NRM ? pp[ip] : (k == 3 ? unp[ip] : 0.0);
-            const double extm = k == NRM ? pp[im] : (k == 3 ? unp[im] : 0.0);
-            lfp[i] = 0.5 * (wp[ip] * U[ip][k] + extp);
-            lfm[i] = 0.5 * (wm[im] * U[im][k] + extm);
+        for (int m = 0; m < 5; m++) {
+            const double ext = k == NRM ? pp[m] : (k == 3 ? unp[m] : 0.0);
+            lf[m] = 0.5 * (w[m] * U[m][k] + ext);
         }
-        weno5z_vjp(lfp, lam_h[k], lp);
-        weno5z_vjp(lfm, lam_h[k], lm);
+        weno5z_vjp(lf, lam_h[k], ll);
 #pragma unroll
-        for (int i = 0; i < 5; i++) {
-            out[(long long)((0 * 5 + i) * 4 + k) * nf + fo] = lp[i];
-            out[(long long)((1 * 5 + i) * 4 + k) * nf + fo] = lm[i];
-        }
+        for (int m = 0; m < 5; m++) out[(long long)((sign * 5 + m) * 4 + k) * nf + fo] = ll[m];
     }
 }
 
+// Four threads per cell, one per (direction, sign): gather the 5 stencil
+// adjoints of the cell's split flux from the faces whose stencils hold it (a
+// fixed order: deterministic), apply the transpose of the split-flux Jacobian,
+// then sum the four parts in a fixed order and apply the RK combination.
+// Block = 64 cells x 4 parts; part = threadIdx.x / 64.
 __global__ void __launch_bounds__(256) adj_cell_kernel(AdjArgs a)
 {
+    __shared__ double part_l[4][4][64];
     const int ni = a.ni, nj = a.nj;
     const long long nf = (long long)ni * nj;
-    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= nf) return;
-    const int j = (int)(idx / ni), i = (int)(idx - (long long)j * ni);
+    const int part = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long long idx = (long long)blockIdx.x * 64 + lane;
+    const bool live = idx < nf;
+    const int j = live ? (int)(idx / ni) : 0, i = live ? (int)(idx - (long long)j * ni) : 0;
     const long long o = (long long)j * a.ld + i;
-    double U[4], lU[4] = {0, 0, 0, 0};
+    double lU[4] = {0, 0, 0, 0};
+    if (live) {
+        double U[4];
 #pragma unroll
-    for (int k = 0; k < 4; k++) U[k] = a.U[k * a.plane + o];
-    // x: + stencil of face f covers cells f-3..f+1 (index i - f + 3), - stencil f+2..f-2 (index f + 2 - i)
-    double lF[4];
+        for (int k = 0; k < 4; k++) U[k] = a.U[k * a.plane + o];
+        const CellPrim cp = cell_prim(a.gamma, U);
+        const bool ydir = part >= 2;
+        const int sign = part & 1;
+        const double* lam = ydir ? a.lamy : a.lamx;
+        double lF[4];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        double sum = 0.0;
+        for (int k = 0; k < 4; k++) {
+            double sum = 0.0;
 #pragma unroll
-        for (int m = 0; m < 5; m++) sum += a.lamx[(long long)((0 * 5 + m) * 4 + k) * nf + (long long)j * ni + wrapi(i + 3 - m, ni)];
-        lF[k] = sum;
+            for (int m = 0; m < 5; m++) {
+                // + sign: this cell is lf(m) of face i+3-m; - sign: of face i-2+m
+                const int f = sign ? m - 2 : 3 - m;
+                const long long fo = ydir ? (long long)wrapi(j + f, nj) * ni + i : (long long)j * ni + wrapi(i + f, ni);
+                sum += lam[(long long)((sign * 5 + m) * 4 + k) * nf + fo];
+            }
+            lF[k] = sum;
+        }
+        if (ydir) lf_flux_vjp<1>(a.gamma, U, cp, sign ? -1.0 : 1.0, lF, lU);
+        else lf_flux_vjp<0>(a.gamma, U, cp, sign ? -1.0 : 1.0, lF, lU);
     }
-    lf_flux_vjp<0>(a.gamma, U, 1.0, lF, lU);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        double sum = 0.0;
-#pragma unroll
-        for (int m = 0; m < 5; m++) sum += a.lamx[(long long)((1 * 5 + m) * 4 + k) * nf + (long long)j * ni + wrapi(i - 2 + m, ni)];
-        lF[k] = sum;
-    }
-    lf_flux_vjp<0>(a.gamma, U, -1.0, lF, lU);
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        double sum = 0.0;
-#pragma unroll
-        for (int m = 0; m < 5; m++) sum += a.lamy[(long long)((0 * 5 + m) * 4 + k) * nf + (long long)wrapi(j + 3 - m, nj) * ni + i];
-        lF[k] = sum;
-    }
-    lf_flux_vjp<1>(a.gamma, U, 1.0, lF, lU);
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        double sum = 0.0;
-#pragma unroll
-        for (int m = 0; m < 5; m++) sum += a.lamy[(long long)((1 * 5 + m) * 4 + k) * nf + (long long)wrapi(j - 2 + m, nj) * ni + i];
-        lF[k] = sum;
-    }
-    lf_flux_vjp<1>(a.gamma, U, -1.0, lF, lU);
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const double g = a.g[k * a.plane + o];
-        const double acc = a.use_acc_in ? a.acc_in[k * a.plane + o] : 0.0;
-        a.g_out[k * a.plane + o] = a.add_acc ? a.bcoef * g + lU[k] + acc : a.bcoef * g + lU[k];
-        if (a.write_acc) a.acc_out[k * a.plane + o] = acc + a.acoef * g;
-    }
+    for (int k = 0; k < 4; k++) part_l[part][k][lane] = lU[k];
+    __syncthreads();
+    // thread (k = part, lane): component k of cell lane
+    if (!live) return;
+    const int k = part;
+    const double l = ((part_l[0][k][lane] + part_l[1][k][lane]) + part_l[2][k][lane]) + part_l[3][k][lane];
+    const double g = a.g[k * a.plane + o];
+    const double acc = a.use_acc_in ? a.acc_in[k * a.plane + o] : 0.0;
+    a.g_out[k * a.plane + o] = a.add_acc ? a.bcoef * g + l + acc : a.bcoef * g + l;
+    if (a.write_acc) a.acc_out[k * a.plane + o] = acc + a.acoef * g;
 }
 
 // fresh wave-speed ring and queues before a forward or recomputed segment
@@ -260,7 +275,8 @@ using namespace dense;
 struct dense_adjoint {
     dense_cfg cfg{};
     int scheme = -1, nsteps = 0, k = 0, nseg = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // forward pass and the reverse sweep
+    hipStream_t rstream = nullptr;  // recomputation of the next segment, overlapped with the reverse sweep
     int ld = 0, tiles_x = 0, ntiles = 0;
     long long plane = 0;
     DevState* st = nullptr;
@@ -269,12 +285,14 @@ struct dense_adjoint {
     std::vector<double> h_dts;
     double* U = nullptr;     // forward work buffers: 3 states
     double* ckpt = nullptr;  // nseg states
-    double* seg = nullptr;   // (3k+1) states of the segment being reversed
+    double* seg[2] = {nullptr, nullptr};  // stage states (3k+1 each) of two segments: one reversed, one recomputed
     double* gb = nullptr;    // g, g_next, acc (3 states)
     double* lam = nullptr;   // 80 face planes
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ready[2] = {nullptr, nullptr}, freed[2] = {nullptr, nullptr}, join = nullptr;
     double fwd_ms = 0.0, bwd_ms = 0.0;
     bool have_forward = false;
+    bool last_seg_fresh = false;  // seg[(nseg-1)&1] holds the last segment's stage states from the forward pass
     char err[512] = {0};
 };
 
@@ -326,7 +344,8 @@ static StageArgs adj_stage_args(const dense_adjoint* a)
 }
 
 // one forward step U0 -> (U1, U2) -> U3 with explicit buffers
-static int adj_enqueue_step(dense_adjoint* a, double* U0, double* U1, double* U2, double* U3, const double* sched)
+static int adj_enqueue_step(dense_adjoint* a, hipStream_t strm, double* U0, double* U1, double* U2, double* U3,
+                            const double* sched)
 {
     StageArgs s = adj_stage_args(a);
     s.dt_sched = sched;
@@ -338,20 +357,43 @@ static int adj_enqueue_step(dense_adjoint* a, double* U0, double* U1, double* U2
         s.un = U0;
         s.us = ins[k - 1];
         s.out = outs[k - 1];
-        ACHK(a, launch_stage(a->scheme, a->cfg.tile, s, a->stream));
+        ACHK(a, launch_stage(a->scheme, a->cfg.tile, s, strm));
     }
     return DENSE_OK;
 }
 
 // fresh reduction ring + wave speeds of U0 (the stage kernels read them)
-static int adj_prime(dense_adjoint* a, double* U0, double t, int step, int step_end)
+static int adj_prime(dense_adjoint* a, hipStream_t strm, double* U0, double t, int step, int step_end)
 {
-    hipLaunchKernelGGL(adj_reset_kernel, dim3(1), dim3(1), 0, a->stream, a->st, 4096);
+    hipLaunchKernelGGL(adj_reset_kernel, dim3(1), dim3(1), 0, strm, a->st, 4096);
     ACHK(a, hipGetLastError());
     StageArgs s = adj_stage_args(a);
     s.us = U0;
-    ACHK(a, launch_max(s, a->stream));
-    ACHK(a, launch_set_clock(a->st, t, INFINITY, step, step_end, a->stream));
+    ACHK(a, launch_max(s, strm));
+    ACHK(a, launch_set_clock(a->st, t, INFINITY, step, step_end, strm));
+    return DENSE_OK;
+}
+
+static void seg_range(const dense_adjoint* a, int sgi, int& n0, int& len)
+{
+    n0 = sgi * a->k;
+    len = (n0 + a->k <= a->nsteps) ? a->k : a->nsteps - n0;
+}
+
+// stage states of segment sgi into seg[sgi&1] from its checkpoint, on the recompute stream
+static int adj_recompute(dense_adjoint* a, int sgi)
+{
+    const size_t st_bytes = sizeof(double) * 4 * a->plane;
+    double* S = a->seg[sgi & 1];
+    int n0, len;
+    seg_range(a, sgi, n0, len);
+    ACHK(a, hipMemcpyAsync(S, state_at(a->ckpt, a, sgi), st_bytes, hipMemcpyDeviceToDevice, a->rstream));
+    int rc = adj_prime(a, a->rstream, S, 0.0, n0, n0 + len);
+    if (rc) return rc;
+    for (int j = 0; j < len; j++)
+        if ((rc = adj_enqueue_step(a, a->rstream, state_at(S, a, 3 * j), state_at(S, a, 3 * j + 1),
+                                   state_at(S, a, 3 * j + 2), state_at(S, a, 3 * j + 3), a->d_dts)))
+            return rc;
     return DENSE_OK;
 }
 
@@ -385,18 +427,26 @@ extern "C" int dense_adj_create(const dense_cfg* cfg, int nsteps, int checkpoint
     a->ntiles = tile_count(c.tile, c.ni, c.nj, &a->tiles_x);
     const size_t st_bytes = sizeof(double) * 4 * a->plane;
     const size_t nf = (size_t)c.ni * c.nj;
-    if (hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&a->rstream, hipStreamNonBlocking) != hipSuccess)
         return bail(afail(nullptr, DENSE_ERR_HIP, "stream creation failed"));
     for (auto& e : a->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail(afail(nullptr, DENSE_ERR_HIP, "event creation failed"));
+    for (int b = 0; b < 2; b++)
+        if (hipEventCreateWithFlags(&a->ready[b], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&a->freed[b], hipEventDisableTiming) != hipSuccess)
+            return bail(afail(nullptr, DENSE_ERR_HIP, "event creation failed"));
+    if (hipEventCreateWithFlags(&a->join, hipEventDisableTiming) != hipSuccess)
+        return bail(afail(nullptr, DENSE_ERR_HIP, "event creation failed"));
+    const size_t seg_bytes = (size_t)(3 * a->k + 1) * st_bytes;
     if (hipMalloc(&a->st, sizeof(DevState)) != hipSuccess || hipMalloc(&a->dt_hist, sizeof(double) * 4096) != hipSuccess ||
         hipMalloc(&a->d_dts, sizeof(double) * nsteps) != hipSuccess || hipMalloc(&a->U, 3 * st_bytes) != hipSuccess ||
         hipMalloc(&a->ckpt, (size_t)a->nseg * st_bytes) != hipSuccess ||
-        hipMalloc(&a->seg, (size_t)(3 * a->k + 1) * st_bytes) != hipSuccess ||
+        hipMalloc(&a->seg[0], seg_bytes) != hipSuccess || hipMalloc(&a->seg[1], seg_bytes) != hipSuccess ||
         hipMalloc(&a->gb, 3 * st_bytes) != hipSuccess || hipMalloc(&a->lam, sizeof(double) * 80 * nf) != hipSuccess)
         return bail(afail(nullptr, DENSE_ERR_NOMEM, "adjoint allocation failed"));
-    if (hipMemset(a->U, 0, 3 * st_bytes) != hipSuccess || hipMemset(a->seg, 0, (size_t)(3 * a->k + 1) * st_bytes) != hipSuccess ||
-        hipMemset(a->gb, 0, 3 * st_bytes) != hipSuccess)
+    if (hipMemset(a->U, 0, 3 * st_bytes) != hipSuccess || hipMemset(a->seg[0], 0, seg_bytes) != hipSuccess ||
+        hipMemset(a->seg[1], 0, seg_bytes) != hipSuccess || hipMemset(a->gb, 0, 3 * st_bytes) != hipSuccess)
         return bail(afail(nullptr, DENSE_ERR_HIP, "memset failed"));
     a->h_dts.assign(nsteps, 0.0);
     *out = a;
@@ -407,13 +457,20 @@ extern "C" int dense_adj_destroy(dense_adj_handle a)
 {
     if (!a) return DENSE_OK;
     if (a->stream) (void)hipStreamSynchronize(a->stream);
-    double* bufs[] = {a->dt_hist, a->d_dts, a->U, a->ckpt, a->seg, a->gb, a->lam};
+    if (a->rstream) (void)hipStreamSynchronize(a->rstream);
+    double* bufs[] = {a->dt_hist, a->d_dts, a->U, a->ckpt, a->seg[0], a->seg[1], a->gb, a->lam};
     for (double* p : bufs)
         if (p) (void)hipFree(p);
     if (a->st) (void)hipFree(a->st);
     for (auto& e : a->ev)
         if (e) (void)hipEventDestroy(e);
+    for (int b = 0; b < 2; b++) {
+        if (a->ready[b]) (void)hipEventDestroy(a->ready[b]);
+        if (a->freed[b]) (void)hipEventDestroy(a->freed[b]);
+    }
+    if (a->join) (void)hipEventDestroy(a->join);
     if (a->stream) (void)hipStreamDestroy(a->stream);
+    if (a->rstream) (void)hipStreamDestroy(a->rstream);
     delete a;
     return DENSE_OK;
 }
@@ -433,24 +490,40 @@ extern "C" int dense_adj_forward(dense_adj_handle a, const double* u0_dev, doubl
     if (!a || !u0_dev || !un_dev) return afail(a, DENSE_ERR_ARG, "null argument");
     const dense_cfg& c = a->cfg;
     const size_t st_bytes = sizeof(double) * 4 * a->plane;
+    a->have_forward = false;
+    a->last_seg_fresh = false;
     int rc = copy_packed(a, a->U, a->ld, u0_dev, c.ni);
     if (rc) return rc;
     if (dt_schedule)
         ACHK(a, hipMemcpyAsync(a->d_dts, dt_schedule, sizeof(double) * a->nsteps, hipMemcpyHostToDevice, a->stream));
-    if ((rc = adj_prime(a, a->U, 0.0, 0, a->nsteps))) return rc;
+    if ((rc = adj_prime(a, a->stream, a->U, 0.0, 0, a->nsteps))) return rc;
+    // the last segment's stage states are kept (in the buffer its reversal reads), the
+    // others are recomputed from checkpoints in the backward pass
+    const int last = a->nseg - 1;
+    int ln0, llen;
+    seg_range(a, last, ln0, llen);
+    double* S = a->seg[last & 1];
+    const double* sched = dt_schedule ? a->d_dts : nullptr;
     ACHK(a, hipEventRecord(a->ev[0], a->stream));
     for (int n = 0; n < a->nsteps; n++) {
         if (n % a->k == 0)
-            ACHK(a, hipMemcpyAsync(state_at(a->ckpt, a, n / a->k), a->U, st_bytes, hipMemcpyDeviceToDevice, a->stream));
-        // stage 3 writes back in place (it reads U^n only at the cell it writes)
-        if ((rc = adj_enqueue_step(a, a->U, state_at(a->U, a, 1), state_at(a->U, a, 2), a->U,
-                                   dt_schedule ? a->d_dts : nullptr)))
-            return rc;
+            ACHK(a, hipMemcpyAsync(state_at(a->ckpt, a, n / a->k), a->U, st_bytes,
+                                   hipMemcpyDeviceToDevice, a->stream));
+        if (n < ln0) {
+            // stage 3 writes back in place (it reads U^n only at the cell it writes)
+            rc = adj_enqueue_step(a, a->stream, a->U, state_at(a->U, a, 1), state_at(a->U, a, 2), a->U, sched);
+        } else {
+            const int j = n - ln0;
+            if (j == 0) ACHK(a, hipMemcpyAsync(S, a->U, st_bytes, hipMemcpyDeviceToDevice, a->stream));
+            rc = adj_enqueue_step(a, a->stream, state_at(S, a, 3 * j), state_at(S, a, 3 * j + 1),
+                                  state_at(S, a, 3 * j + 2), state_at(S, a, 3 * j + 3), sched);
+        }
+        if (rc) return rc;
     }
     ACHK(a, hipEventRecord(a->ev[1], a->stream));
     if (!dt_schedule)
         ACHK(a, hipMemcpyAsync(a->d_dts, a->dt_hist, sizeof(double) * a->nsteps, hipMemcpyDeviceToDevice, a->stream));
-    if ((rc = copy_packed(a, un_dev, c.ni, a->U, a->ld))) return rc;
+    if ((rc = copy_packed(a, un_dev, c.ni, state_at(S, a, 3 * llen), a->ld))) return rc;
     ACHK(a, hipMemcpyAsync(a->h_dts.data(), a->d_dts, sizeof(double) * a->nsteps, hipMemcpyDeviceToHost, a->stream));
     DevState sh{};
     ACHK(a, hipMemcpyAsync(&sh, a->st, sizeof sh, hipMemcpyDeviceToHost, a->stream));
@@ -459,8 +532,9 @@ extern "C" int dense_adj_forward(dense_adj_handle a, const double* u0_dev, doubl
     ACHK(a, hipEventElapsedTime(&ms, a->ev[0], a->ev[1]));
     a->fwd_ms = ms;
     if (dts_out) std::memcpy(dts_out, a->h_dts.data(), sizeof(double) * a->nsteps);
-    a->have_forward = true;
     if (sh.error) return afail(a, DENSE_ERR_NONFINITE, "non-finite wave speed in the forward pass");
+    a->have_forward = true;
+    a->last_seg_fresh = true;
     return DENSE_OK;
 }
 
@@ -489,9 +563,9 @@ static int adj_vjp(dense_adjoint* a, const double* Us, const double* g, double* 
     x.add_acc = add_acc;
     x.write_acc = acc_out != nullptr;
     const long long nf = (long long)c.ni * c.nj;
-    hipLaunchKernelGGL(adj_face_kernel, dim3((unsigned)((2 * nf + 255) / 256)), dim3(256), 0, a->stream, x);
+    hipLaunchKernelGGL(adj_face_kernel, dim3((unsigned)((nf + 255) / 256), 4), dim3(256), 0, a->stream, x);
     ACHK(a, hipGetLastError());
-    hipLaunchKernelGGL(adj_cell_kernel, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, a->stream, x);
+    hipLaunchKernelGGL(adj_cell_kernel, dim3((unsigned)((nf + 63) / 64)), dim3(256), 0, a->stream, x);
     ACHK(a, hipGetLastError());
     return DENSE_OK;
 }
@@ -499,34 +573,42 @@ static int adj_vjp(dense_adjoint* a, const double* Us, const double* g, double* 
 extern "C" int dense_adj_backward(dense_adj_handle a, const double* gn_dev, double* g0_dev)
 {
     if (!a || !gn_dev || !g0_dev) return afail(a, DENSE_ERR_ARG, "null argument");
-    if (!a->have_forward) return afail(a, DENSE_ERR_ARG, "dense_adj_backward before dense_adj_forward");
+    if (!a->have_forward) return afail(a, DENSE_ERR_ARG, "dense_adj_backward without a successful dense_adj_forward");
     const dense_cfg& c = a->cfg;
-    const size_t st_bytes = sizeof(double) * 4 * a->plane;
     double* g = state_at(a->gb, a, 0);
     double* gn = state_at(a->gb, a, 1);
     double* acc = state_at(a->gb, a, 2);
     int rc = copy_packed(a, g, a->ld, gn_dev, c.ni);
     if (rc) return rc;
     ACHK(a, hipEventRecord(a->ev[2], a->stream));
-    for (int sgi = a->nseg - 1; sgi >= 0; sgi--) {
-        const int n0 = sgi * a->k;
-        const int len = (n0 + a->k <= a->nsteps) ? a->k : a->nsteps - n0;
-        // recompute the segment's stage states from its checkpoint, same kernels and step sizes
-        ACHK(a, hipMemcpyAsync(state_at(a->seg, a, 0), state_at(a->ckpt, a, sgi), st_bytes, hipMemcpyDeviceToDevice,
-                               a->stream));
-        if ((rc = adj_prime(a, state_at(a->seg, a, 0), 0.0, n0, n0 + len))) return rc;
-        for (int j = 0; j < len; j++) {
-            double* U0 = state_at(a->seg, a, 3 * j);
-            if ((rc = adj_enqueue_step(a, U0, state_at(a->seg, a, 3 * j + 1), state_at(a->seg, a, 3 * j + 2),
-                                       state_at(a->seg, a, 3 * j + 3), a->d_dts)))
-                return rc;
+    ACHK(a, hipStreamWaitEvent(a->rstream, a->ev[2], 0));
+    // Segment sgi is reversed on `stream` from seg[sgi&1] while segment sgi-1 is
+    // recomputed on `rstream` into the other buffer; events order the hand-offs.
+    // The last segment's states come from the forward pass (same stream) unless
+    // a backward pass has run since.
+    const int last = a->nseg - 1;
+    bool used[2] = {false, false};
+    const bool fresh = a->last_seg_fresh;
+    if (!fresh) {
+        if ((rc = adj_recompute(a, last))) return rc;
+        ACHK(a, hipEventRecord(a->ready[last & 1], a->rstream));
+    }
+    for (int sgi = last; sgi >= 0; sgi--) {
+        if (sgi > 0) {
+            const int b = (sgi - 1) & 1;
+            if (used[b]) ACHK(a, hipStreamWaitEvent(a->rstream, a->freed[b], 0));
+            if ((rc = adj_recompute(a, sgi - 1))) return rc;
+            ACHK(a, hipEventRecord(a->ready[b], a->rstream));
         }
-        // reverse through the segment
+        if (sgi != last || !fresh) ACHK(a, hipStreamWaitEvent(a->stream, a->ready[sgi & 1], 0));
+        double* S = a->seg[sgi & 1];
+        int n0, len;
+        seg_range(a, sgi, n0, len);
         for (int j = len - 1; j >= 0; j--) {
             const double dt = a->h_dts[n0 + j];
-            double* U0 = state_at(a->seg, a, 3 * j);
-            double* U1 = state_at(a->seg, a, 3 * j + 1);
-            double* U2 = state_at(a->seg, a, 3 * j + 2);
+            double* U0 = state_at(S, a, 3 * j);
+            double* U1 = state_at(S, a, 3 * j + 1);
+            double* U2 = state_at(S, a, 3 * j + 2);
             // stage 3: U3 = 1/3 U0 + 2/3 U2 + 2/3 dt L(U2)
             if ((rc = adj_vjp(a, U2, g, gn, nullptr, acc, false, 2.0 / 3.0, 2.0 / 3.0, 1.0 / 3.0, dt))) return rc;
             // stage 2: U2 = 3/4 U0 + 1/4 U1 + 1/4 dt L(U1)
@@ -535,8 +617,13 @@ extern "C" int dense_adj_backward(dense_adj_handle a, const double* gn_dev, doub
             if ((rc = adj_vjp(a, U0, g, gn, acc, nullptr, true, 1.0, 1.0, 0.0, dt))) return rc;
             std::swap(g, gn);
         }
+        ACHK(a, hipEventRecord(a->freed[sgi & 1], a->stream));
+        used[sgi & 1] = true;
     }
+    ACHK(a, hipEventRecord(a->join, a->rstream));
+    ACHK(a, hipStreamWaitEvent(a->stream, a->join, 0));
     ACHK(a, hipEventRecord(a->ev[3], a->stream));
+    a->last_seg_fresh = false;  // its buffer may have been reused
     if ((rc = copy_packed(a, g0_dev, c.ni, g, a->ld))) return rc;
     ACHK(a, hipStreamSynchronize(a->stream));
     float ms = 0.f;
