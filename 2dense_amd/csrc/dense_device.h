@@ -22,21 +22,85 @@ constexpr double C30 = 0.1, C31 = 0.6, C32 = 0.3;
 
 enum Recon : int { R_UP5 = 0, R_JS = 1, R_Z = 2 };
 
+// ---------------------------------------------------------------- division
+// gfx950 has no f64 divide; the compiler expands a/b into
+//   s0 = div_scale(b, b, a); y = rcp(s0) refined by two Newton steps
+//   (e = fma(-s0, y, 1), y = fma(y, e, y), twice); s1 = div_scale(a, b, a);
+//   q0 = s1*y; r = fma(-s0, q0, s1); q = div_fmas(r, y, q0); div_fixup(q, b, a).
+// Both div_scale steps return their operand unscaled and leave the scale flag
+// clear, and div_fixup returns its input, when the operands and the quotient
+// are normal numbers away from the exponent limits:
+//   2^-969 <= |a|, b normal with |b| < 2^1021, 2^-1022 <= |a/b| < 2^767,
+// or a = +-0 with b > 0 (div_fixup returns a zero of a's sign).  In that range
+// a/b is, bit for bit, y = nr_recip(b); q = div_by(a, b, y), so a divisor
+// shared by several quotients needs one reciprocal.  Every use below tests a
+// sufficient range condition per quotient and recomputes that quotient with
+// '/' where it fails (a short divergent branch, taken on extreme data only).
+__device__ __forceinline__ double nr_recip(double b)
+{
+    const double r = __builtin_amdgcn_rcp(b);
+    const double e0 = fma(-b, r, 1.0);
+    const double y1 = fma(r, e0, r);
+    const double e1 = fma(-b, y1, 1.0);
+    return fma(y1, e1, y1);
+}
+
+// The residual is formed negated, fma(b, q0, -a) = -fma(-b, q0, a) exactly,
+// so that a = -0 yields -0 like IEEE division (identical bits otherwise).
+__device__ __forceinline__ double div_by(double a, double b, double y)
+{
+    const double q0 = a * y;
+    const double rn = fma(b, q0, -a);
+    return fma(-rn, y, q0);
+}
+
+// n/12 for n >= 0 (the smoothness indicators).  The residual is formed at
+// scale 2^128: for 2^-969 <= n <= 2^890 this is the sequence above with an
+// exact residual scaled by a power of two (identical bits); for tiny n
+// (2^-1018 <= n < 2^-969, where the compiler's sequence scales both operands
+// by 2^128) it is that scaled sequence, q = RN(n*2^128/12)*2^-128 = RN(n/12),
+// the quotient being normal.  n = 0 gives +0.  Other n take '/'.
+// 12 is hidden from constant folding so that its reciprocal is formed at run
+// time by the same instructions as in the compiler's n/12.0.
+__device__ __forceinline__ double div12(double n)
+{
+    double twelve = 12.0;
+    asm("" : "+v"(twelve));
+    const double y = nr_recip(twelve);
+    const double q0 = n * y;
+    const double rn = fma(twelve * 0x1p128, q0, -(n * 0x1p128));
+    double q = fma(-rn, y * 0x1p-128, q0);
+    if (__builtin_expect(!(n == 0.0 || (n >= 0x1p-1018 && n <= 0x1p890)), 0)) q = n / 12.0;
+    return q;
+}
+
 // ---------------------------------------------------------------- stencils
 
+// The stencils take FAST = true in the characteristic WENO-Z scheme (divisions
+// are about half of its instructions; measured 3-4 % faster per stage) and
+// FAST = false ('/') elsewhere: the component-wise schemes run at the register
+// limit of their occupancy, and AdaWENO measured no gain.
+template <bool FAST>
+__device__ __forceinline__ double over12(double n)
+{
+    if constexpr (FAST) return div12(n);
+    else return n / 12.0;
+}
+
 // smoothness indicators, weno_mod.f90:152-159 / :175-182
+template <bool FAST>
 __device__ __forceinline__ void smooth_ind(const double* lf, double& is0, double& is1, double& is2)
 {
     double x, y;
     x = (lf[0] - 2.0 * lf[1]) + lf[2];
     y = (lf[0] - 4.0 * lf[1]) + 3.0 * lf[2];
-    is0 = 13.0 * (x * x) / 12.0 + (y * y) / 4.0;
+    is0 = over12<FAST>(13.0 * (x * x)) + (y * y) / 4.0;
     x = (lf[1] - 2.0 * lf[2]) + lf[3];
     y = lf[1] - lf[3];
-    is1 = 13.0 * (x * x) / 12.0 + (y * y) / 4.0;
+    is1 = over12<FAST>(13.0 * (x * x)) + (y * y) / 4.0;
     x = (lf[2] - 2.0 * lf[3]) + lf[4];
     y = (3.0 * lf[2] - 4.0 * lf[3]) + lf[4];
-    is2 = 13.0 * (x * x) / 12.0 + (y * y) / 4.0;
+    is2 = over12<FAST>(13.0 * (x * x)) + (y * y) / 4.0;
 }
 
 // candidate polynomials, weno_mod.f90:57-59
@@ -47,11 +111,36 @@ __device__ __forceinline__ void candidates(const double* lf, double& q0, double&
     q2 = A320 * lf[2] + A321 * lf[3] + A322 * lf[4];
 }
 
+// w_k = a_k / s for a_k = c_k(...) >= c_k >= 0.1 and s = sum a_k: one shared
+// reciprocal when s <= 2^1000 (then 0.1 <= a_k <= s and the quotients lie in
+// [2^-1004, 1]); '/' otherwise (s infinite or NaN).
+template <bool FAST>
+__device__ __forceinline__ void normalize3(double aa0, double aa1, double aa2, double s, double& w0, double& w1,
+                                           double& w2)
+{
+    if constexpr (!FAST) {
+        w0 = aa0 / s;
+        w1 = aa1 / s;
+        w2 = aa2 / s;
+        return;
+    }
+    const double ys = nr_recip(s);
+    w0 = div_by(aa0, s, ys);
+    w1 = div_by(aa1, s, ys);
+    w2 = div_by(aa2, s, ys);
+    if (__builtin_expect(!(s <= 0x1p1000), 0)) {
+        w0 = aa0 / s;
+        w1 = aa1 / s;
+        w2 = aa2 / s;
+    }
+}
+
 // WENO-Z non-linear weights (+ AdaWENO indicator theta), weno_mod.f90:67-95,168-188
+template <bool FAST>
 __device__ __forceinline__ void weno5z_omega(const double* lf, double& w0, double& w1, double& w2, double* theta)
 {
     double is0, is1, is2;
-    smooth_ind(lf, is0, is1, is2);
+    smooth_ind<FAST>(lf, is0, is1, is2);
     const double tau5 = fabs(is2 - is0);
     double t;
     t = tau5 / (WENO_SS + is0);
@@ -61,9 +150,7 @@ __device__ __forceinline__ void weno5z_omega(const double* lf, double& w0, doubl
     t = tau5 / (WENO_SS + is2);
     const double aa2 = C32 * (1.0 + t * t);
     const double s = aa0 + aa1 + aa2;
-    w0 = aa0 / s;
-    w1 = aa1 / s;
-    w2 = aa2 / s;
+    normalize3<FAST>(aa0, aa1, aa2, s, w0, w1, w2);
     if (theta) {
         const double d = s - 1.0;
         *theta = 1.0 / (1.0 + d * d);
@@ -71,28 +158,48 @@ __device__ __forceinline__ void weno5z_omega(const double* lf, double& w0, doubl
 }
 
 // weno5z_stencil, weno_mod.f90:50-65
+template <bool FAST>
 __device__ __forceinline__ double weno5z(const double* lf)
 {
     double q0, q1, q2, w0, w1, w2;
     candidates(lf, q0, q1, q2);
-    weno5z_omega(lf, w0, w1, w2, nullptr);
+    weno5z_omega<FAST>(lf, w0, w1, w2, nullptr);
     return w0 * q0 + w1 * q1 + w2 * q2;
 }
 
-// weno5js_stencil, weno_mod.f90:98-165
+// weno5js_stencil, weno_mod.f90:98-165.  FAST: a_k = c_k/(eps+beta_k)^2 through
+// reciprocals while (eps+beta_k)^2 <= 2^900 (then a_k in [2^-904, 2^40] and
+// sum <= 2^42), '/' otherwise.
+template <bool FAST>
 __device__ __forceinline__ double weno5js(const double* lf)
 {
     double q0, q1, q2, is0, is1, is2, s;
     candidates(lf, q0, q1, q2);
-    smooth_ind(lf, is0, is1, is2);
+    smooth_ind<FAST>(lf, is0, is1, is2);
     s = WENO_SS + is0;
-    const double aa0 = C30 / (s * s);
+    const double d0 = s * s;
     s = WENO_SS + is1;
-    const double aa1 = C31 / (s * s);
+    const double d1 = s * s;
     s = WENO_SS + is2;
-    const double aa2 = C32 / (s * s);
+    const double d2 = s * s;
+    double aa0, aa1, aa2;
+    if constexpr (FAST) {
+        aa0 = div_by(C30, d0, nr_recip(d0));
+        aa1 = div_by(C31, d1, nr_recip(d1));
+        aa2 = div_by(C32, d2, nr_recip(d2));
+        if (__builtin_expect(!(d0 <= 0x1p900 && d1 <= 0x1p900 && d2 <= 0x1p900), 0)) {
+            aa0 = C30 / d0;
+            aa1 = C31 / d1;
+            aa2 = C32 / d2;
+        }
+    } else {
+        aa0 = C30 / d0;
+        aa1 = C31 / d1;
+        aa2 = C32 / d2;
+    }
     const double sum = aa0 + aa1 + aa2;
-    const double w0 = aa0 / sum, w1 = aa1 / sum, w2 = aa2 / sum;
+    double w0, w1, w2;
+    normalize3<FAST>(aa0, aa1, aa2, sum, w0, w1, w2);
     return w0 * q0 + w1 * q1 + w2 * q2;
 }
 
@@ -106,8 +213,8 @@ template <int R>
 __device__ __forceinline__ double recon(const double* lf)
 {
     if constexpr (R == R_UP5) return upwind5(lf);
-    else if constexpr (R == R_JS) return weno5js(lf);
-    else return weno5z(lf);
+    else if constexpr (R == R_JS) return weno5js<false>(lf);
+    else return weno5z<false>(lf);
 }
 
 // ---------------------------------------------------------------- cells

@@ -106,7 +106,7 @@ __device__ __forceinline__ void face_char(const double* __restrict__ s, int base
             const double un = s[VN * NC + o];
             lfi[i] = s[o] + 0.5 * ((un + alpha) * s[NRM * NC + o] + s[REC_P * NC + o]);
         }
-        weno5z_omega(lfi, wp0, wp1, wp2, &th);
+        weno5z_omega<false>(lfi, wp0, wp1, wp2, &th);
         chp = !(th > delta);
 #pragma unroll
         for (int i = 0; i < 5; i++) {
@@ -114,7 +114,7 @@ __device__ __forceinline__ void face_char(const double* __restrict__ s, int base
             const double un = s[VN * NC + o];
             lfi[i] = s[o] + 0.5 * ((un - alpha) * s[NRM * NC + o] + s[REC_P * NC + o]);
         }
-        weno5z_omega(lfi, wm0, wm1, wm2, &th);
+        weno5z_omega<false>(lfi, wm0, wm1, wm2, &th);
         chm = !(th > delta);
     }
 
@@ -185,7 +185,7 @@ __device__ __forceinline__ void face_char(const double* __restrict__ s, int base
                     for (int i = 0; i < 5; i++)
                         lf[i] = l4[0] * F[i][0] + l4[1] * F[i][1] + l4[2] * F[i][2] + l4[3] * F[i][3];
                 }
-                const double cv = weno5z(lf);
+                const double cv = weno5z<!ADAPT>(lf);
                 // back-projection h(c) = sum_k r(k,c) cv_k, accumulated in k order
                 if (kc == 0) {
                     acc_r = cv;
