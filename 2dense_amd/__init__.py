@@ -119,7 +119,7 @@ def make_cfg(ni: int, nj: int, *, benchmark: int = 0, splitting: int = LLF, reco
              stages: int = 3, if_const_dt: bool = False, if_bodyforce: bool | None = None,
              dx: float | None = None, dy: float | None = None, bc=None, bc_state=None,
              tile: int = 0) -> Cfg:
-    """Build a dense_cfg.  For benchmark 1/3/5/9 the grid metrics and boundary
+    """Build a dense_cfg.  For benchmark 1/2/3/4/5/9 the grid metrics and boundary
     rules come from the reference's canned problem (dense_problem_defaults;
     benchmark 3 also sets gamma = 5/3 and the gravity body force, as
     init_RT_general.f90:9-10 does); for benchmark 0 give dx, dy, bc (4 kinds)

@@ -124,6 +124,18 @@ extern "C" int dense_problem_defaults(int benchmark, int ni, int nj, dense_cfg* 
         c->bc[0] = c->bc[1] = DENSE_BC_EXTRAP;
         c->bc[2] = c->bc[3] = DENSE_BC_REFLECT;
         return DENSE_OK;
+    case 2:  // Sedov blast, init_sedov_general.f90:13-16: reflective left/bottom, outlet right/top
+        c->dx = 1.2 / ni;
+        c->dy = 1.2 / nj;
+        c->bc[DENSE_SIDE_LEFT] = c->bc[DENSE_SIDE_BOTTOM] = DENSE_BC_REFLECT;
+        c->bc[DENSE_SIDE_RIGHT] = c->bc[DENSE_SIDE_TOP] = DENSE_BC_EXTRAP;
+        return DENSE_OK;
+    case 4:  // Richtmyer-Meshkov, init_RM_general.f90:12-15: outlet left/right, reflective bottom/top
+        c->dx = 4.0 / ni;
+        c->dy = 1.0 / nj;
+        c->bc[0] = c->bc[1] = DENSE_BC_EXTRAP;
+        c->bc[2] = c->bc[3] = DENSE_BC_REFLECT;
+        return DENSE_OK;
     case 0:
         return DENSE_OK;
     default:

@@ -3,8 +3,9 @@
 ! Reads the reference's input.in (the same six NAMELIST groups, variable
 ! names and switch codes, flow_mod_general.f90:51-180, read in the order of
 ! read_par_namelist, :313-332), builds one of the canned problems
-! (benchmark 1 isentropic vortex, 3 Rayleigh-Taylor, 5 double Mach
-! reflection, 9 shock/vortex), advances it on the GPU through the C ABI and
+! (benchmark 1 isentropic vortex, 2 Sedov blast, 3 Rayleigh-Taylor,
+! 4 Richtmyer-Meshkov, 5 double Mach reflection, 9 shock/vortex), advances
+! it on the GPU through the C ABI and
 ! writes the reference's outputs: per-step stdout, 2d_<step>.dat2d binary
 ! dumps, 2d_<step>.dat Tecplot files and CpuTime.dat (general_main.f90:1-63).
 !
@@ -224,10 +225,60 @@ contains
           call put_state(i, j, rho, u, v, p)
         end do
       end do
+    case (2)  ! Sedov blast: init_sedov_general.f90:13-67
+      dx = 1.2d0 / ni
+      dy = 1.2d0 / nj
+      call cell_centres(0.5d0)
+      p = e_blast()
+      do j = 1, nj
+        do i = 1, ni
+          call put_state(i, j, 1.d0, 0.d0, 0.d0, dble(1.0e-6))
+        end do
+      end do
+      call put_state(1, 1, 1.d0, 0.d0, 0.d0, p)
+    case (4)  ! Richtmyer-Meshkov: init_RM_general.f90:12-64
+      dx = 4.d0 / ni
+      dy = 1.d0 / nj
+      call cell_centres(0.5d0)
+      do j = 1, nj
+        do i = 1, ni
+          x = xc(i, j)
+          y = yc(i, j)
+          if (x < dble(0.05)) then
+            call put_state(i, j, dble(1.376364), dble(0.394729), 0.d0, dble(1.569800))
+          else if (x < (1.d0 - dble(0.05) * sin(2 * pie * (y - 0.25d0))) .and. x > dble(0.05)) then
+            call put_state(i, j, 1.d0, 0.d0, 0.d0, 1.d0)
+          else
+            call put_state(i, j, dble(0.1358), 0.d0, 0.d0, 1.d0)
+          end if
+        end do
+      end do
     case default
       print *, 'No such Benchmark index!'
       stop 1
     end select
+  contains
+    ! cell centres (i - h)*dx, (j - h)*dy and vertices (i - 1)*dx, (j - 1)*dy
+    subroutine cell_centres(h)
+      double precision, intent(in) :: h
+      integer :: ii, jj
+      do jj = 1, nj
+        do ii = 1, ni
+          xc(ii, jj) = (ii - h) * dx
+          yc(ii, jj) = (jj - h) * dy
+        end do
+      end do
+      do jj = 1, nj + 1
+        do ii = 1, ni + 1
+          vet_x(ii, jj) = (ii - 1.d0) * dx
+          vet_y(ii, jj) = (jj - 1.d0) * dy
+        end do
+      end do
+    end subroutine cell_centres
+    ! p_0 = e (gamma - 1) with e = 0.244816 / (dx dy), init_sedov_general.f90:36-37
+    double precision function e_blast()
+      e_blast = 0.244816d0 / (dx * dy) * (gamma - 1.d0)
+    end function e_blast
   end subroutine init_problem
 
   ! Switches and metrics handed to the library.

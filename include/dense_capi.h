@@ -32,7 +32,7 @@ extern "C" {
 #define DENSE_ERR_NONFINITE (-4)   /* non-finite wave speed: the state has blown up */
 #define DENSE_ERR_NOMEM (-5)       /* device allocation failed */
 
-/* Boundary sides and kinds.  benchmark_index 1, 3, 5, 9 select the reference's
+/* Boundary sides and kinds.  benchmark_index 1, 2, 3, 4, 5, 9 select the reference's
  * own boundary routines (boundary_general.f90:5-31); benchmark 0 takes the
  * per-side description in dense_cfg.bc / bc_state. */
 enum { DENSE_SIDE_LEFT = 0, DENSE_SIDE_RIGHT = 1, DENSE_SIDE_BOTTOM = 2, DENSE_SIDE_TOP = 3 };
@@ -61,7 +61,7 @@ typedef struct dense_cfg {
     int stages;              /* STAGES: 1, 2 or 3 */
     int if_const_dt;         /* IF_CONST_DT */
     int if_bodyforce;        /* gravity source of benchmark 3 (bodyforce = (0,0,rho,rho v)) */
-    int benchmark;           /* BENCHMARK_INDEX: 1, 3, 5, 9, or 0 for bc[] below */
+    int benchmark;           /* BENCHMARK_INDEX: 1, 2, 3, 4, 5, 9, or 0 for bc[] below */
     int bc[4];               /* per-side kind (benchmark 0) */
     double bc_state[4][4];   /* per-side Dirichlet (rho,u,v,p); DMR sides: post-shock state */
     int tile;                /* tile shape index, 0 = default */
@@ -71,7 +71,7 @@ typedef struct dense_solver* dense_handle;
 
 /* Fill cfg->dx, dy, bc, bc_state (and gamma, if_bodyforce for benchmark 3) for
  * one of the reference's canned problems: init_general.f90:13-37.
- * Replaces the metric part of init_{IVC,RT,doublemach,shockvortex}_general.f90. */
+ * Replaces the metric part of init_{IVC,sedov,RT,RM,doublemach,shockvortex}_general.f90. */
 int dense_problem_defaults(int benchmark, int ni, int nj, dense_cfg* cfg);
 
 /* Allocate device state for one block.

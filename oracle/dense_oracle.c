@@ -1107,6 +1107,22 @@ int ora_problem_params(int benchmark, int ni, int nj, ora_params* p)
         p->bc[0] = p->bc[1] = ORA_BC_EXTRAP;
         p->bc[2] = p->bc[3] = ORA_BC_REFLECT;
         return 0;
+    case 2: /* init_sedov_general.f90:13-16; boundary_sedov_general.f90 (bottom :13-44 and
+               left :68-100 reflective, top :46-66 and right :101-121 outlet) */
+        p->dx = 1.2 / ni;
+        p->dy = 1.2 / nj;
+        p->bc[0] = ORA_BC_REFLECT;
+        p->bc[1] = ORA_BC_EXTRAP;
+        p->bc[2] = ORA_BC_REFLECT;
+        p->bc[3] = ORA_BC_EXTRAP;
+        return 0;
+    case 4: /* init_RM_general.f90:12-15; boundary_rm_general.f90 (bottom :15-45 and top :47-79
+               reflective, left :81-101 and right :103-123 outlet) */
+        p->dx = 4.0 / ni;
+        p->dy = 1.0 / nj;
+        p->bc[0] = p->bc[1] = ORA_BC_EXTRAP;
+        p->bc[2] = p->bc[3] = ORA_BC_REFLECT;
+        return 0;
     default:
         return -1;
     }
@@ -1207,6 +1223,28 @@ int ora_init_state(ora_block* b, int benchmark)
                     p = 1.245000;
                 }
                 put_cell(b, i, j, rho, u, v, p);
+            }
+        return 0;
+    }
+    case 2: { /* init_sedov_general.f90:33-67: quiescent gas, the blast energy in the corner cell */
+        const double e = 0.244816 / (dx * dy);
+        const double p0 = e * (gamma - 1.0), p1 = SP(1.0e-6);
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) put_cell(b, i, j, 1.0, 0.0, 0.0, p1);
+        put_cell(b, 1, 1, 1.0, 0.0, 0.0, p0);
+        return 0;
+    }
+    case 4: { /* init_RM_general.f90:33-64: shocked gas, unperturbed gas, light gas behind a
+                 sinusoidal interface x = 1 - 0.05 sin(2 pi (y - 0.25)) */
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) {
+                double x = ((double)i - 0.5) * dx, y = ((double)j - 0.5) * dy;
+                if (x < SP(0.05))
+                    put_cell(b, i, j, SP(1.376364), SP(0.394729), 0.0, SP(1.569800));
+                else if (x < (1.0 - SP(0.05) * sin(2 * pie * (y - 0.25))) && x > SP(0.05))
+                    put_cell(b, i, j, 1.0, 0.0, 0.0, 1.0);
+                else
+                    put_cell(b, i, j, SP(0.1358), 0.0, 0.0, 1.0);
             }
         return 0;
     }

@@ -8,7 +8,7 @@ the per-step dt it printed.  These files are data (inputs = the switches in
 `CASES`, outputs = the reference's numbers); run this script in the build
 container, where the reference sources exist.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case names]   # all cases, or only those named
 """
 import json
 import os
@@ -44,23 +44,32 @@ CASES += [
     ("sv_101_255", 9, 64, 32, 101, 255, 0.5, 10, {}),
     ("sv_104", 9, 64, 32, 104, 255, 0.5, 10, {}),
     ("sv_105", 9, 64, 32, 105, 255, 0.5, 10, {}),
+    ("sedov_101_255", 2, 48, 48, 101, 255, 0.5, 10, {}),
+    ("sedov_104", 2, 48, 48, 104, 255, 0.5, 10, {}),
+    ("sedov_103_253", 2, 40, 40, 103, 253, 0.5, 10, {}),
+    ("rm_101_253", 4, 64, 16, 101, 253, 0.5, 10, {}),
+    ("rm_104", 4, 64, 16, 104, 255, 0.5, 10, {}),
+    ("rm_105", 4, 64, 16, 105, 255, 0.5, 10, {}),
 ]
 
 
-def main():
+def main(only=None):
+    """only: names of the cases to (re)write; the index lists every case."""
     index = {}
     for name, bm, ni, nj, sp, rc, cfl, steps, extra in CASES:
         nml = dict(benchmark=bm, ni=ni, nj=nj, splitting=sp, recon=rc, cfl=cfl)
         nml.update(extra)
+        index[name] = dict(nml, steps=steps)
+        if only and name not in only:
+            continue
         d = po.run_reference(steps=steps, **nml)
         cons = d["cons"][:, po.BF:-po.BF, po.BF:-po.BF]
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), cons=cons, dts=d["dts"],
                             time=np.float64(d["time"]), step=np.int64(d["step"]))
-        index[name] = dict(nml, steps=steps)
         print(name, cons.shape, "t =", d["time"])
     with open(os.path.join(HERE, "cases.json"), "w") as f:
         json.dump(index, f, indent=1, sort_keys=True)
 
 
 if __name__ == "__main__":
-    main()
+    main(set(sys.argv[1:]) or None)
