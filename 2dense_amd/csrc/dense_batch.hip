@@ -399,6 +399,8 @@ extern "C" int dense_batch_create(const dense_cfg* cfgs, int n, dense_batch_hand
         char why[256];
         int rc = validate_cfg(&c, why, sizeof why);
         if (rc) return bail(bfail(nullptr, rc, "instance %d: %s", i, why));
+        if (c.bc[DENSE_SIDE_LEFT] == DENSE_BC_SSL_INLET)  // needs host math at every step
+            return bail(bfail(nullptr, DENSE_ERR_UNSUPPORTED, "instance %d: shear-layer inflow not batched", i));
         const int sc = scheme_of(c.splitting, c.recon);
         if (i == 0) {
             b->scheme = sc;

@@ -32,8 +32,20 @@ struct dense_solver {
     double last_run_ms = 0.0;
     int last_run_launches = 0;
     int tiles_x = 0, ntiles = 0;
+    double* bc_rows = nullptr;    // device: shear-layer inflow tables (2 nj), else null
+    double* bct_host = nullptr;   // pinned: clock terms of the shear-layer inflow
     char err[512] = {0};
 };
+
+// Clock terms of the shear-layer inflow at t, with the host's libm as the
+// reference evaluates them (boundary_ssl_general.f90:70-71):
+// 0.05 cos(2 pi t/(30/2.68)) and 0.05 cos(2 pi 2 t/(30/2.68) + pi/2).
+static void ssl_clock_terms(double t, double* out)
+{
+    const double pie = std::atan(1.0) * 4.0;
+    out[0] = 0.05 * std::cos(2.0 * pie * t / (30.0 / 2.68));
+    out[1] = 0.05 * std::cos(2.0 * pie * 2.0 * t / (30.0 / 2.68) + pie / 2.0);
+}
 
 static thread_local char g_err[512];
 
@@ -67,7 +79,9 @@ int validate_cfg(const dense_cfg* c, char* why, size_t n)
                DENSE_ERR_UNSUPPORTED;
     if (c->tile < 0 || c->tile > 3) return snprintf(why, n, "tile index out of range"), DENSE_ERR_ARG;
     for (int s = 0; s < 4; s++)
-        if (c->bc[s] < 0 || c->bc[s] > 5) return snprintf(why, n, "bad boundary kind on side %d", s), DENSE_ERR_ARG;
+        if (c->bc[s] < 0 || c->bc[s] > 6) return snprintf(why, n, "bad boundary kind on side %d", s), DENSE_ERR_ARG;
+    for (int s = 1; s < 4; s++)
+        if (c->bc[s] == DENSE_BC_SSL_INLET) return snprintf(why, n, "shear-layer inflow only on the left"), DENSE_ERR_ARG;
     if ((c->bc[0] == DENSE_BC_PERIODIC) != (c->bc[1] == DENSE_BC_PERIODIC) ||
         (c->bc[2] == DENSE_BC_PERIODIC) != (c->bc[3] == DENSE_BC_PERIODIC))
         return snprintf(why, n, "periodic boundaries must be paired"), DENSE_ERR_ARG;
@@ -130,6 +144,20 @@ extern "C" int dense_problem_defaults(int benchmark, int ni, int nj, dense_cfg* 
         c->bc[DENSE_SIDE_LEFT] = c->bc[DENSE_SIDE_BOTTOM] = DENSE_BC_REFLECT;
         c->bc[DENSE_SIDE_RIGHT] = c->bc[DENSE_SIDE_TOP] = DENSE_BC_EXTRAP;
         return DENSE_OK;
+    case 6:  // shear layer, init_ssl_general.f90:12-15: inflow left (boundary_ssl_general.f90:55-85),
+             // outlet right, reflective bottom; the top ghost cells keep the state the
+             // initialisation gives them (init_ssl_general.f90:62-81; the top loop of
+             // boundary_ssl_general.f90:47-52 is empty): a fixed state
+        c->dx = 200.0 / ni;
+        c->dy = 40.0 / nj;
+        c->bc[DENSE_SIDE_LEFT] = DENSE_BC_SSL_INLET;
+        c->bc_state[DENSE_SIDE_LEFT][0] = 40.0 / 2.0;
+        c->bc[DENSE_SIDE_RIGHT] = DENSE_BC_EXTRAP;
+        c->bc[DENSE_SIDE_BOTTOM] = DENSE_BC_REFLECT;
+        c->bc[DENSE_SIDE_TOP] = DENSE_BC_DIRICHLET;
+        c->bc_state[DENSE_SIDE_TOP][0] = 2.1101, c->bc_state[DENSE_SIDE_TOP][1] = 2.9709;
+        c->bc_state[DENSE_SIDE_TOP][2] = -0.1367, c->bc_state[DENSE_SIDE_TOP][3] = (double)0.4754f;
+        return DENSE_OK;
     case 4:  // Richtmyer-Meshkov, init_RM_general.f90:12-15: outlet left/right, reflective bottom/top
         c->dx = 4.0 / ni;
         c->dy = 1.0 / nj;
@@ -168,6 +196,7 @@ static StageArgs base_args(const dense_solver* h)
         a.bc.kind[s] = c.bc[s];
         for (int v = 0; v < 4; v++) a.bc.state[s][v] = c.bc_state[s][v];
     }
+    a.bc_rows = h->bc_rows;
     return a;
 }
 
@@ -249,6 +278,21 @@ extern "C" int dense_create(const dense_cfg* cfg, dense_handle* out)
     if (hipMemcpyAsync(h->st, &z, sizeof z, hipMemcpyHostToDevice, h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess)
         return bail(fail(nullptr, DENSE_ERR_HIP, "state init failed"));
+    if (c.bc[DENSE_SIDE_LEFT] == DENSE_BC_SSL_INLET) {
+        // per-row terms of the inflow, boundary_ssl_general.f90:58-71, with the
+        // cell-centre y of init_ssl_general.f90:20 (host libm, as the reference)
+        std::vector<double> rows(2 * (size_t)c.nj);
+        for (int j = 0; j < c.nj; j++) {
+            const double y = ((double)(j + 1) - 0.5) * c.dy - c.bc_state[DENSE_SIDE_LEFT][0];
+            rows[j] = 2.5 + 0.5 * std::tanh(2 * y);
+            rows[c.nj + j] = std::exp(-(y * y) / 10.0);
+        }
+        if (hipMalloc(&h->bc_rows, sizeof(double) * rows.size()) != hipSuccess ||
+            hipHostMalloc(&h->bct_host, sizeof(double) * 2) != hipSuccess)
+            return bail(fail(nullptr, DENSE_ERR_NOMEM, "inflow table allocation failed"));
+        if (hipMemcpy(h->bc_rows, rows.data(), sizeof(double) * rows.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(nullptr, DENSE_ERR_HIP, "inflow table upload failed"));
+    }
     *out = h;
     return DENSE_OK;
 }
@@ -268,6 +312,8 @@ extern "C" int dense_destroy(dense_handle h)
     if (h->stage_host) (void)hipHostFree(h->stage_host);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->bc_rows) (void)hipFree(h->bc_rows);
+    if (h->bct_host) (void)hipHostFree(h->bct_host);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return DENSE_OK;
@@ -321,6 +367,13 @@ extern "C" int dense_get_state(dense_handle h, double* cons, double* prim)
         return fail(h, DENSE_ERR_NOMEM, "padded buffer allocation failed");
     if (!h->pad_host && hipHostMalloc(&h->pad_host, sizeof(double) * 9 * np) != hipSuccess)
         return fail(h, DENSE_ERR_NOMEM, "pinned padded buffer allocation failed");
+    if (h->bc_rows) {
+        // inflow ghost cells at the solver's clock
+        int rc = read_state(h);
+        if (rc) return rc;
+        ssl_clock_terms(h->st_host->t, h->bct_host);
+        HIPCHK(h, hipMemcpyAsync(&h->st->bc_t[0], h->bct_host, sizeof(double) * 2, hipMemcpyHostToDevice, h->stream));
+    }
     StageArgs a = base_args(h);
     a.us = h->U[0];
     HIPCHK(h, launch_fill_padded(a, h->pad_dev, h->pad_dev + 4 * np, h->stream));
@@ -345,7 +398,9 @@ extern "C" int dense_get_state(dense_handle h, double* cons, double* prim)
 extern "C" int dense_step(dense_handle h, double current_time, double* dt)
 {
     if (!h) return fail(h, DENSE_ERR_ARG, "null handle");
-    HIPCHK(h, launch_set_clock(h->st, current_time, INFINITY, 0, 1, h->stream));
+    double tc[2] = {0.0, 0.0};
+    if (h->bc_rows) ssl_clock_terms(current_time, tc);
+    HIPCHK(h, launch_set_clock(h->st, current_time, INFINITY, 0, 1, h->stream, tc[0], tc[1]));
     int rc = enqueue_step(h);
     if (rc) return rc;
     rc = read_state(h);
@@ -382,6 +437,35 @@ extern "C" int dense_run(dense_handle h, int max_steps, double total_time, int t
     if (!h || !t || !step || max_steps < 0) return fail(h, DENSE_ERR_ARG, "bad arguments");
     if (steps_taken) *steps_taken = 0;
     if (max_steps == 0) return DENSE_OK;
+    if (h->bc_rows) {
+        // the inflow needs the host's cos of t^n: one synchronised step at a time,
+        // the clock advanced as the device advances it (t + dt)
+        double tt = *t;
+        int st = *step, taken = 0;
+        HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+        while (taken < max_steps && tt < total_time && st < total_step) {
+            double dt = 0.0;
+            int rc = dense_step(h, tt, &dt);
+            if (rc) return rc;
+            if (dt_hist) dt_hist[taken] = dt;
+            tt = tt + dt;
+            st += 1;
+            taken += 1;
+        }
+        HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        // leave the device clock where the step loop left it
+        HIPCHK(h, launch_set_clock(h->st, tt, INFINITY, st, st, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        *t = tt;
+        *step = st;
+        if (steps_taken) *steps_taken = taken;
+        float ms = 0.f;
+        HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        h->last_run_ms = ms;
+        h->last_run_launches = taken * h->cfg.stages;
+        return DENSE_OK;
+    }
     const int chunk = 16;
     if (h->graph_steps != chunk) {
         int rc = build_graph(h, chunk);

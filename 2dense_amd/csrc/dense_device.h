@@ -330,6 +330,17 @@ __device__ __forceinline__ void ghost_cell(const StageArgs& a, const double* __r
         }
         return;
     }
+    case DENSE_BC_SSL_INLET: {
+        // boundary_ssl_general.f90:55-85 at t^n: rho, p by the sign of y;
+        // u = 2.5 + 0.5 tanh(2y) and exp(-y^2/10) tabulated per row by the host,
+        // 0.05 cos(.) of the two forcing modes in st->bc_t (host libm)
+        const double y = ((double)(gy + 1) - 0.5) * a.dy - sv[0];
+        const double e = a.bc_rows[a.nj + gy];
+        const double v = a.st->bc_t[0] * e + a.st->bc_t[1] * e;
+        if (y >= 0.0) from_state(a.gamma, 1.6374, a.bc_rows[gy], v, 0.3327, c, pr);
+        else from_state(a.gamma, 0.3626, a.bc_rows[gy], v, 0.3327, c, pr);
+        return;
+    }
     case DENSE_BC_DMR_TOP: {
         // boundary_doublemach_general.f90:67-80, shock at t^n
         const double x = ((double)(gx + 1) - 0.5) * a.dx;

@@ -46,6 +46,7 @@ struct alignas(64) DevState {
     int error;                       // 1: non-finite wave speed
     int hist_n;                      // dt history entries written
     int hist_cap;
+    double bc_t[2];                  // clock terms of the shear-layer inflow at t^n (set by the host)
 };
 
 struct BcDesc {
@@ -69,6 +70,7 @@ struct StageArgs {
     int last_stage;
     int tiles_x, ntiles;
     BcDesc bc;
+    const double* bc_rows;  // shear-layer inflow: u(y_j) for j < nj, then exp(-y_j^2/10)
 };
 
 // ---- batched mode: many independent blocks in one launch per RK stage
@@ -109,7 +111,8 @@ hipError_t launch_batch_max(const BatchArgs& b, hipStream_t s);
 // host-side launchers (dense_kernels.hip)
 hipError_t launch_stage(int scheme, int tile, const StageArgs& a, hipStream_t s);
 hipError_t launch_max(const StageArgs& a, hipStream_t s);
-hipError_t launch_set_clock(DevState* st, double t, double t_end, int step, int step_end, hipStream_t s);
+hipError_t launch_set_clock(DevState* st, double t, double t_end, int step, int step_end, hipStream_t s,
+                            double bc_t0 = 0.0, double bc_t1 = 0.0);
 hipError_t launch_fill_padded(const StageArgs& a, double* cons_pad, double* prim_pad, hipStream_t s);
 int tile_count(int tile, int ni, int nj, int* tiles_x);
 

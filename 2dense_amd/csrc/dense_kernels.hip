@@ -62,8 +62,11 @@ __global__ void __launch_bounds__(256) max_kernel(StageArgs a)
     launch_epilogue(st, seq, false, 0.0, 0.0, nullptr);
 }
 
-__global__ void set_clock_kernel(DevState* st, double t, double t_end, int step, int step_end)
+__global__ void set_clock_kernel(DevState* st, double t, double t_end, int step, int step_end, double bc_t0,
+                                 double bc_t1)
 {
+    st->bc_t[0] = bc_t0;
+    st->bc_t[1] = bc_t1;
     st->t = t;
     st->t_end = t_end;
     st->step = step;
@@ -173,9 +176,10 @@ hipError_t launch_max(const StageArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_set_clock(DevState* st, double t, double t_end, int step, int step_end, hipStream_t s)
+hipError_t launch_set_clock(DevState* st, double t, double t_end, int step, int step_end, hipStream_t s,
+                            double bc_t0, double bc_t1)
 {
-    hipLaunchKernelGGL(set_clock_kernel, dim3(1), dim3(1), 0, s, st, t, t_end, step, step_end);
+    hipLaunchKernelGGL(set_clock_kernel, dim3(1), dim3(1), 0, s, st, t, t_end, step, step_end, bc_t0, bc_t1);
     return hipGetLastError();
 }
 

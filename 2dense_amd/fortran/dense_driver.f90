@@ -4,7 +4,8 @@
 ! names and switch codes, flow_mod_general.f90:51-180, read in the order of
 ! read_par_namelist, :313-332), builds one of the canned problems
 ! (benchmark 1 isentropic vortex, 2 Sedov blast, 3 Rayleigh-Taylor,
-! 4 Richtmyer-Meshkov, 5 double Mach reflection, 9 shock/vortex), advances
+! 4 Richtmyer-Meshkov, 5 double Mach reflection, 6 shear layer, 9
+! shock/vortex), advances
 ! it on the GPU through the C ABI and
 ! writes the reference's outputs: per-step stdout, 2d_<step>.dat2d binary
 ! dumps, 2d_<step>.dat Tecplot files and CpuTime.dat (general_main.f90:1-63).
@@ -252,6 +253,34 @@ contains
             call put_state(i, j, dble(0.1358), 0.d0, 0.d0, 1.d0)
           end if
         end do
+      end do
+    case (6)  ! shear layer: init_ssl_general.f90:12-81
+      dx = 200.d0 / ni
+      dy = 40.d0 / nj
+      do j = 1, nj
+        do i = 1, ni
+          xc(i, j) = (i - 0.5d0) * dx
+          yc(i, j) = (j - 0.5d0) * dy - 40.d0 / 2.0
+        end do
+      end do
+      do j = 1, nj + 1
+        do i = 1, ni + 1
+          vet_x(i, j) = (i - 1.0d0) * dx
+          vet_y(i, j) = (j - 1.0d0) * dy - 40.d0 / 2.0
+        end do
+      end do
+      do j = 1, nj
+        do i = 1, ni
+          if (yc(i, j) >= 0) then
+            call put_state(i, j, 1.6374d0, 3.d0, 0.d0, 0.3327d0)
+          else
+            call put_state(i, j, 0.3626d0, 2.d0, 0.d0, 0.3327d0)
+          end if
+        end do
+      end do
+      ! the top row in the state of the top boundary (whose ghost rows the library holds fixed)
+      do i = 1, ni
+        call put_state(i, nj, 2.1101d0, 2.9709d0, -0.1367d0, dble(0.4754))
       end do
     case default
       print *, 'No such Benchmark index!'

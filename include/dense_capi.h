@@ -32,7 +32,7 @@ extern "C" {
 #define DENSE_ERR_NONFINITE (-4)   /* non-finite wave speed: the state has blown up */
 #define DENSE_ERR_NOMEM (-5)       /* device allocation failed */
 
-/* Boundary sides and kinds.  benchmark_index 1, 2, 3, 4, 5, 9 select the reference's
+/* Boundary sides and kinds.  benchmark_index 1, 2, 3, 4, 5, 6, 9 select the reference's
  * own boundary routines (boundary_general.f90:5-31); benchmark 0 takes the
  * per-side description in dense_cfg.bc / bc_state. */
 enum { DENSE_SIDE_LEFT = 0, DENSE_SIDE_RIGHT = 1, DENSE_SIDE_BOTTOM = 2, DENSE_SIDE_TOP = 3 };
@@ -42,7 +42,9 @@ enum {
     DENSE_BC_REFLECT = 2,   /* boundary_shockvortex_general.f90:15-79 */
     DENSE_BC_DIRICHLET = 3, /* boundary_rt_general.f90:14-57, boundary_doublemach_general.f90:98-118 */
     DENSE_BC_DMR_WALL = 4,  /* boundary_doublemach_general.f90:15-63 (bottom only) */
-    DENSE_BC_DMR_TOP = 5    /* boundary_doublemach_general.f90:65-96 (top only) */
+    DENSE_BC_DMR_TOP = 5,   /* boundary_doublemach_general.f90:65-96 (top only) */
+    DENSE_BC_SSL_INLET = 6  /* boundary_ssl_general.f90:55-85, shear-layer inflow (left only);
+                               bc_state[left][0] = yl/2, cell centre y = (j - 1/2) dy - yl/2 */
 };
 
 /* Run configuration: the NAMELIST variables the hot path reads
@@ -61,7 +63,7 @@ typedef struct dense_cfg {
     int stages;              /* STAGES: 1, 2 or 3 */
     int if_const_dt;         /* IF_CONST_DT */
     int if_bodyforce;        /* gravity source of benchmark 3 (bodyforce = (0,0,rho,rho v)) */
-    int benchmark;           /* BENCHMARK_INDEX: 1, 2, 3, 4, 5, 9, or 0 for bc[] below */
+    int benchmark;           /* BENCHMARK_INDEX: 1, 2, 3, 4, 5, 6, 9, or 0 for bc[] below */
     int bc[4];               /* per-side kind (benchmark 0) */
     double bc_state[4][4];   /* per-side Dirichlet (rho,u,v,p); DMR sides: post-shock state */
     int tile;                /* tile shape index, 0 = default */
@@ -71,7 +73,7 @@ typedef struct dense_solver* dense_handle;
 
 /* Fill cfg->dx, dy, bc, bc_state (and gamma, if_bodyforce for benchmark 3) for
  * one of the reference's canned problems: init_general.f90:13-37.
- * Replaces the metric part of init_{IVC,sedov,RT,RM,doublemach,shockvortex}_general.f90. */
+ * Replaces the metric part of init_{IVC,sedov,RT,RM,doublemach,ssl,shockvortex}_general.f90. */
 int dense_problem_defaults(int benchmark, int ni, int nj, dense_cfg* cfg);
 
 /* Allocate device state for one block.
@@ -102,6 +104,9 @@ int dense_get_state(dense_handle h, double* cons, double* prim);
  * receives the step size (CFL or constant, time_advance_general.f90:20-24);
  * the caller advances its own clock as general_main.f90:35-37 does. */
 int dense_step(dense_handle h, double current_time, double* dt);
+/* (The shear-layer inflow evaluates cos of t^n: with that boundary, dense_run
+ * steps from the host, one synchronised step at a time, instead of replaying
+ * graphs; results are the same.) */
 
 /* Up to max_steps steps while (t < total_time && step < total_step)
  * (general_main.f90:23-24), replayed from a captured HIP graph with the clock

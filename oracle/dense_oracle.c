@@ -422,6 +422,27 @@ static void boundary(ora_block* b)
                 set_dirichlet(b, i, j, b->p.bc_state[0][0], b->p.bc_state[0][1], b->p.bc_state[0][2],
                               b->p.bc_state[0][3]);
         break;
+    case ORA_BC_SSL_INLET: {
+        /* boundary_ssl_general.f90:55-85, at t = current_time */
+        const double pie = atan(1.0) * 4.0, t = b->current_time;
+        for (j = BF + 1; j <= nj + BF; j++)
+            for (i = 1; i <= BF; i++) {
+                const double y = ((double)(j - BF) - 0.5) * b->p.dy - b->p.bc_state[0][0];
+                double rho, p;
+                if (y >= 0.0) {
+                    rho = 1.6374;
+                    p = 0.3327;
+                } else {
+                    rho = 0.3626;
+                    p = 0.3327;
+                }
+                const double u = 2.5 + 0.5 * tanh(2 * y);
+                const double v = 0.05 * cos(2.0 * pie * t / (30.0 / 2.68)) * exp(-(y * y) / 10.0) +
+                                 0.05 * cos(2.0 * pie * 2.0 * t / (30.0 / 2.68) + pie / 2.0) * exp(-(y * y) / 10.0);
+                set_dirichlet(b, i, j, rho, u, v, p);
+            }
+        break;
+    }
     }
     /* right (x+) */
     switch (b->p.bc[1]) {
@@ -1116,6 +1137,19 @@ int ora_problem_params(int benchmark, int ni, int nj, ora_params* p)
         p->bc[2] = ORA_BC_REFLECT;
         p->bc[3] = ORA_BC_EXTRAP;
         return 0;
+    case 6: /* init_ssl_general.f90:12-15; boundary_ssl_general.f90 (bottom :15-45 reflective,
+               top :47-52 an empty loop -- the ghost rows keep the initial state, a fixed
+               Dirichlet state --, left :55-85 forced inflow, right :87-107 outlet) */
+        p->dx = 200.0 / ni;
+        p->dy = 40.0 / nj;
+        p->bc[0] = ORA_BC_SSL_INLET;
+        p->bc_state[0][0] = 40.0 / 2.0;
+        p->bc[1] = ORA_BC_EXTRAP;
+        p->bc[2] = ORA_BC_REFLECT;
+        p->bc[3] = ORA_BC_DIRICHLET;
+        p->bc_state[3][0] = 2.1101; p->bc_state[3][1] = 2.9709; p->bc_state[3][2] = -0.1367;
+        p->bc_state[3][3] = SP(0.4754);
+        return 0;
     case 4: /* init_RM_general.f90:12-15; boundary_rm_general.f90 (bottom :15-45 and top :47-79
                reflective, left :81-101 and right :103-123 outlet) */
         p->dx = 4.0 / ni;
@@ -1232,6 +1266,19 @@ int ora_init_state(ora_block* b, int benchmark)
         for (j = 1; j <= nj; j++)
             for (i = 1; i <= ni; i++) put_cell(b, i, j, 1.0, 0.0, 0.0, p1);
         put_cell(b, 1, 1, 1.0, 0.0, 0.0, p0);
+        return 0;
+    }
+    case 6: { /* init_ssl_general.f90:33-81: two streams meeting at y = 0; the top row and the
+                 top ghost rows in the state of the top boundary */
+        for (j = 1; j <= nj; j++)
+            for (i = 1; i <= ni; i++) {
+                double y = ((double)j - 0.5) * dy - 40.0 / 2.0;
+                if (y >= 0)
+                    put_cell(b, i, j, 1.6374, 3.0, 0.0, 0.3327);
+                else
+                    put_cell(b, i, j, 0.3626, 2.0, 0.0, 0.3327);
+            }
+        for (i = 1; i <= ni; i++) put_cell(b, i, nj, 2.1101, 2.9709, -0.1367, SP(0.4754));
         return 0;
     }
     case 4: { /* init_RM_general.f90:33-64: shocked gas, unperturbed gas, light gas behind a

@@ -22,7 +22,8 @@ enum {
     ORA_BC_REFLECT = 2,    /* mirror with normal momentum / velocity negated */
     ORA_BC_DIRICHLET = 3,  /* fixed primitive state */
     ORA_BC_DMR_WALL = 4,   /* bottom of double Mach: Dirichlet post-shock for x<1/6, else reflect */
-    ORA_BC_DMR_TOP = 5     /* top of double Mach: moving shock at x = 1/6 + (1+20 t)/sqrt(3) */
+    ORA_BC_DMR_TOP = 5,    /* top of double Mach: moving shock at x = 1/6 + (1+20 t)/sqrt(3) */
+    ORA_BC_SSL_INLET = 6   /* left of the shear layer: forced inflow; bc_state[0][0] = yl/2 */
 };
 
 typedef struct ora_params {

@@ -50,6 +50,10 @@ CASES += [
     ("rm_101_253", 4, 64, 16, 101, 253, 0.5, 10, {}),
     ("rm_104", 4, 64, 16, 104, 255, 0.5, 10, {}),
     ("rm_105", 4, 64, 16, 105, 255, 0.5, 10, {}),
+    ("ssl_101_255", 6, 64, 32, 101, 255, 0.5, 10, {}),
+    ("ssl_104", 6, 64, 32, 104, 255, 0.5, 10, {}),
+    ("ssl_105_30", 6, 80, 16, 105, 255, 0.5, 30, {}),
+    ("ssl_102_253", 6, 64, 32, 102, 253, 0.5, 10, {}),
 ]
 
 

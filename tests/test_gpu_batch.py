@@ -76,3 +76,13 @@ def test_batch_autotune_restores_state():
     for i in range(len(cfgs)):
         ref, _ = _alone(cfgs[i], states[i], 5)
         assert np.array_equal(got[i], ref)
+
+
+def test_batch_rejects_shear_layer_inflow():
+    """The shear-layer inflow needs the host's cos of t^n at every step: not batched."""
+    D = dense()
+    B = _batch_mod()
+    cfg = D.make_cfg(64, 32, benchmark=6, splitting=D.LLF, recon=D.WENOZ)
+    with pytest.raises(D.DenseError) as e:
+        B.Batch([cfg])
+    assert e.value.code == -2

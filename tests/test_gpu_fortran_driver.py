@@ -33,7 +33,7 @@ def _run_driver(tmp_path, c, steps, **extra):
 
 
 @pytest.mark.parametrize("name", ["dmr_104", "ivc_101_253", "rt_105", "sv_104", "dmr_102_253_constdt",
-                                  "ivc_101_255_s2", "sedov_104", "rm_105"])
+                                  "ivc_101_255_s2", "sedov_104", "rm_105", "ssl_105_30"])
 def test_driver_matches_reference(tmp_path, name):
     if not os.path.exists(DRIVER):
         pytest.fail("Fortran driver not built (python 2dense_amd/build.py)")

@@ -62,9 +62,11 @@ def test_gpu_tile_shape_invariance(name, tile, oracle):
     assert np.array_equal(got, g["cons"])
 
 
-def test_gpu_run_graph_matches_step(oracle):
-    """dense_run (graph replay, device clock) equals dense_step one by one."""
-    c = CASES["dmr_104"]
+@pytest.mark.parametrize("name", ["dmr_104", "ssl_104"])
+def test_gpu_run_graph_matches_step(oracle, name):
+    """dense_run (graph replay with the device clock; host-stepped for the
+    shear-layer inflow) equals dense_step one by one."""
+    c = CASES[name]
     o = _oracle_ic(oracle, c)
     with _solver(c) as a, _solver(c) as b:
         a.set_state(o.cons())
@@ -76,14 +78,15 @@ def test_gpu_run_graph_matches_step(oracle):
         assert a.current_time == b.current_time
 
 
-def test_gpu_run_stops_at_total_time(oracle):
+@pytest.mark.parametrize("name,total", [("dmr_104", 0.005), ("ssl_104", 3.0)])
+def test_gpu_run_stops_at_total_time(oracle, name, total):
     """Loop condition current_time < total_time (general_main.f90:23)."""
-    c = CASES["dmr_104"]
+    c = CASES[name]
     o = _oracle_ic(oracle, c)
-    o.run(0.005, 10 ** 6)
+    o.run(total, 10 ** 6)
     with _solver(c) as s:
         s.set_state(_oracle_ic(oracle, c).cons())
-        s.run(1000, total_time=0.005)
+        s.run(1000, total_time=total)
         assert s.current_step == o.current_step
         assert s.current_time == o.time
         assert np.array_equal(s.interior(), o.interior())
