@@ -73,7 +73,7 @@ def test_adjoint_vs_torch_reference_gradient(steps, k):
     assert np.array_equal(R.dts, dts)
     J = A.density_mismatch(uN, target.cuda(), ic.h, ic.h)
     J.backward()
-    assert float(J) == pytest.approx(float(J_ref), rel=1e-13)
+    assert J.detach().item() == pytest.approx(J_ref.detach().item(), rel=1e-13)
     gu = U0d.grad.cpu().numpy()
     gu_ref = g_ref_u0.numpy()
     err_u = np.abs(gu - gu_ref).max() / np.abs(gu_ref).max()

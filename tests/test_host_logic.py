@@ -1,0 +1,168 @@
+"""Host-side logic that runs without a GPU: the canned problems' metrics and
+boundaries (dense_problem_defaults), configuration validation in
+dense_create / dense_batch_create / dense_adj_create (checked before any
+device call), the ensemble and initial-condition builders, and bench.py's
+refusal of more than one GPU."""
+import ctypes as C
+import importlib
+import math
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, dense
+
+
+def _lib():
+    return dense().load_library()
+
+
+def _defaults(benchmark, ni, nj):
+    D = dense()
+    c = D.Cfg()
+    c.gamma = 1.4
+    rc = _lib().dense_problem_defaults(benchmark, ni, nj, C.byref(c))
+    return rc, c
+
+
+@pytest.mark.parametrize("benchmark,lx,ly,kinds", [
+    (1, 10.0, 10.0, (1, 1, 0, 0)),   # IVC: extrapolation x, periodic y
+    (2, 1.2, 1.2, (2, 1, 2, 1)),     # Sedov: reflect left/bottom, outlet right/top
+    (4, 4.0, 1.0, (1, 1, 2, 2)),     # Richtmyer-Meshkov
+    (5, 4.0, 1.0, (3, 1, 4, 5)),     # double Mach
+    (6, 200.0, 40.0, (6, 1, 2, 3)),  # shear layer
+    (9, 2.0, 1.0, (1, 1, 2, 2)),     # shock/vortex
+])
+def test_problem_defaults_metrics_and_kinds(benchmark, lx, ly, kinds):
+    ni, nj = 80, 40
+    rc, c = _defaults(benchmark, ni, nj)
+    assert rc == 0
+    assert c.dx == lx / ni and c.dy == ly / nj
+    assert tuple(c.bc) == kinds
+    assert c.nghost == 4 and c.neq == 4
+
+
+def test_problem_defaults_rt_and_dmr_states():
+    rc, c = _defaults(3, 64, 256)
+    assert rc == 0
+    assert c.gamma == 5.0 / 3.0 and c.if_bodyforce == 1
+    assert c.dx == 0.25 / 63 and c.dy == 1.0 / 255          # vertex-centred RT grid
+    assert list(c.bc_state[2]) == [2.0, 0.0, 0.0, 1.0] and list(c.bc_state[3]) == [1.0, 0.0, 0.0, 2.5]
+    rc, c = _defaults(5, 64, 16)
+    rho, u, v, p = c.bc_state[0]
+    # Mach 10 post-shock state of the double Mach problem: 8.25 at -30 degrees
+    assert rho == 8.0 and p == 116.5
+    assert math.isclose(math.hypot(u, v), 8.25, rel_tol=1e-15)
+    assert math.isclose(math.degrees(math.atan2(v, u)), -30.0, rel_tol=1e-13)
+
+
+def test_problem_defaults_shear_layer_top_state():
+    rc, c = _defaults(6, 64, 32)
+    assert c.bc_state[0][0] == 20.0                           # yl / 2
+    assert list(c.bc_state[3][:3]) == [2.1101, 2.9709, -0.1367]
+    assert c.bc_state[3][3] == float(np.float32(0.4754))      # single-precision literal, widened
+
+
+def test_unknown_benchmark_rejected():
+    for b in (7, 8, 10, -1):
+        rc, _ = _defaults(b, 32, 32)
+        assert rc == -2
+    assert b"benchmark" in _lib().dense_last_error(None)
+
+
+def _create(**kw):
+    D = dense()
+    cfg = D.make_cfg(kw.pop("ni", 32), kw.pop("nj", 32), benchmark=0, dx=0.1, dy=0.1,
+                     bc=kw.pop("bc", [D.BC_PERIODIC] * 4), **kw)
+    h = C.c_void_p()
+    rc = _lib().dense_create(C.byref(cfg), C.byref(h))
+    return rc, _lib().dense_last_error(None).decode()
+
+
+def test_create_rejects_bad_configurations():
+    D = dense()
+    assert _create(splitting=106)[0] == -2
+    assert _create(recon=254)[0] == -2
+    assert _create(stages=4)[0] == -1
+    assert _create(ni=3)[0] == -1
+    assert _create(tile=7)[0] == -1
+    rc, msg = _create(bc=[D.BC_PERIODIC, D.BC_EXTRAP, D.BC_PERIODIC, D.BC_PERIODIC])
+    assert rc == -1 and "paired" in msg
+    rc, msg = _create(bc=[D.BC_DMR_WALL, D.BC_EXTRAP, D.BC_REFLECT, D.BC_REFLECT])
+    assert rc == -1
+    rc, msg = _create(bc=[D.BC_EXTRAP, 6, D.BC_REFLECT, D.BC_REFLECT])
+    assert rc == -1 and "left" in msg
+
+
+def test_batch_and_adjoint_reject_before_allocating():
+    D = dense()
+    B = importlib.import_module("2dense_amd.batch")
+    bad = D.make_cfg(32, 32, benchmark=1, splitting=999)
+    with pytest.raises(D.DenseError) as e:
+        B.Batch([bad])
+    assert e.value.code == -2
+    one = D.make_cfg(32, 32, benchmark=1, stages=1)
+    with pytest.raises(D.DenseError) as e:
+        B.Batch([one])
+    assert e.value.code == -2
+    A = importlib.import_module("2dense_amd.adjoint")
+    lib = A._bind(_lib())
+    h = C.c_void_p()
+    cfg = D.make_cfg(32, 32, benchmark=0, splitting=D.GLF, recon=D.WENOZ, dx=0.1, dy=0.1, bc=[D.BC_PERIODIC] * 4)
+    assert lib.dense_adj_create(C.byref(cfg), 10, 5, C.byref(h)) == -2
+    cfg = D.make_cfg(32, 32, benchmark=0, splitting=D.LLF, recon=D.WENOZ, dx=0.1, dy=0.1, bc=[D.BC_PERIODIC] * 4)
+    assert lib.dense_adj_create(C.byref(cfg), 5000, 5, C.byref(h)) == -1
+
+
+def test_rankine_hugoniot_post_shock_state():
+    B = importlib.import_module("2dense_amd.batch")
+    g = 1.4
+    for m in (1.05, 1.2, 1.5, 3.0):
+        rho2, u2, p2 = B.shock_post_state(m, g)
+        u1 = m * math.sqrt(g)
+        e = lambda rho, u, p: p / (g - 1) + 0.5 * rho * u * u  # noqa: E731
+        assert math.isclose(1.0 * u1, rho2 * u2, rel_tol=1e-14)                  # mass
+        assert math.isclose(u1 * u1 + 1.0, rho2 * u2 * u2 + p2, rel_tol=1e-14)   # momentum
+        assert math.isclose((e(1.0, u1, 1.0) + 1.0) * u1, (e(rho2, u2, p2) + p2) * u2, rel_tol=1e-13)  # energy
+        assert rho2 > 1.0 and p2 > 1.0 and u2 < u1
+
+
+def test_make_ensemble_is_seeded_and_in_range():
+    B = importlib.import_module("2dense_amd.batch")
+    c1, s1, i1 = B.make_ensemble(n=12, seed=5, nmin=16, nmax=64)
+    c2, s2, i2 = B.make_ensemble(n=12, seed=5, nmin=16, nmax=64)
+    assert [(a.ni, a.nj) for a in c1] == [(a.ni, a.nj) for a in c2]
+    assert all(np.array_equal(a, b) for a, b in zip(s1, s2))
+    for cfg, st, inst in zip(c1, s1, i1):
+        assert st.shape == (4, cfg.nj, cfg.ni) and np.all(np.isfinite(st)) and np.all(st[0] > 0)
+        if inst.kind == "vortex":
+            assert 1.0 <= inst.params["eps"] <= 5.0 and cfg.ni == cfg.nj
+        else:
+            assert 1.05 <= inst.params["mach"] <= 1.5 and cfg.nj == max(8, cfg.ni // 2)
+        assert 16 <= cfg.ni <= 64
+
+
+def test_fourier_ic_base_state_and_linearity():
+    import torch
+    A = importlib.import_module("2dense_amd.adjoint")
+    ic = A.FourierIC(32, n_modes=8)
+    z = torch.zeros(8, dtype=torch.float64)
+    U0 = ic(z)
+    rho, u, v, p, h = A.vortex_prim(32)
+    assert np.allclose(U0[0].numpy(), rho, rtol=0, atol=0)
+    assert np.allclose((U0[1] / U0[0]).numpy(), u, rtol=1e-15)
+    th = torch.tensor(A.draw_amplitudes(8), dtype=torch.float64)
+    d1 = ic(th)[0] - U0[0]
+    d2 = ic(2 * th)[0] - U0[0]
+    assert torch.allclose(d2, 2 * d1, rtol=1e-9, atol=1e-14)        # density is linear in theta
+    assert torch.allclose(ic(th)[3] - U0[3], 0.5 * d1 * (ic.u ** 2 + ic.v ** 2), rtol=1e-9, atol=1e-13)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
