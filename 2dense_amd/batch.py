@@ -5,7 +5,9 @@ one persistent launch per RK stage, each with its own grid, boundaries and
 CFL step, and each evolves bit for bit as it would alone through ``Solver``.
 
 ``make_ensemble`` builds the seeded ensemble of isentropic vortices and
-shock/vortex interactions used to measure the batched path.
+shock/vortex interactions used to measure the batched path; its initial
+states are built either here (``vortex_state``, ``shock_vortex_state``) or on
+the device from the instance parameters (``Batch.init_ic``).
 """
 from __future__ import annotations
 
@@ -17,6 +19,16 @@ import numpy as np
 
 from . import (BC_EXTRAP, BC_PERIODIC, BC_REFLECT, BF, Cfg, DenseError, LLF, WENOZ, _dp, load_library,
                make_cfg)
+
+
+IC_VORTEX, IC_SHOCK_VORTEX = 1, 2
+
+
+class IcDesc(C.Structure):
+    """dense_ic of include/dense_capi.h."""
+    _fields_ = [("kind", C.c_int), ("eps", C.c_double), ("xc", C.c_double), ("yc", C.c_double),
+                ("uinf", C.c_double), ("vinf", C.c_double), ("length", C.c_double), ("mach", C.c_double),
+                ("xs", C.c_double), ("rc", C.c_double), ("alpha", C.c_double)]
 
 
 def _bind(lib):
@@ -35,8 +47,10 @@ def _bind(lib):
     lib.dense_batch_last_run_gpu_ms.argtypes = [H, dp]
     lib.dense_batch_last_error.argtypes = [H]
     lib.dense_batch_last_error.restype = C.c_char_p
+    lib.dense_batch_init_ic.argtypes = [H, C.POINTER(IcDesc)]
     for n in ("dense_batch_create", "dense_batch_destroy", "dense_batch_set_state", "dense_batch_get_state",
-              "dense_batch_run", "dense_batch_set_schedule", "dense_batch_autotune", "dense_batch_last_run_gpu_ms"):
+              "dense_batch_run", "dense_batch_set_schedule", "dense_batch_autotune", "dense_batch_last_run_gpu_ms",
+              "dense_batch_init_ic"):
         getattr(lib, n).restype = C.c_int
     lib._batch_bound = True
     return lib
@@ -92,6 +106,17 @@ class Batch:
         self._check(self.lib.dense_batch_set_state(self.h, i, _dp(a), t0), self.h)
         self.t[i] = t0
 
+    def init_ic(self, insts):
+        """Generate every instance's initial state on the device from its
+        parameters (``Instance`` list as made by ``make_ensemble``, or
+        ``IcDesc`` list); clocks restart at 0."""
+        descs = [x if isinstance(x, IcDesc) else ic_desc(x) for x in insts]
+        if len(descs) != self.n:
+            raise ValueError(f"{len(descs)} descriptors for {self.n} instances")
+        arr = (IcDesc * self.n)(*descs)
+        self._check(self.lib.dense_batch_init_ic(self.h, arr), self.h)
+        self.t = np.zeros(self.n)
+
     def interior(self, i: int) -> np.ndarray:
         a = np.empty(self._shape(i))
         self._check(self.lib.dense_batch_get_state(self.h, i, _dp(a)), self.h)
@@ -126,9 +151,11 @@ def cons_from_prim(rho, u, v, p, gamma):
     return np.stack([rho, rho * u, rho * v, p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)])
 
 
-def vortex_state(n: int, eps: float, xc: float, yc: float, gamma: float = 1.4, L: float = 10.0):
+def vortex_state(n: int, eps: float, xc: float, yc: float, gamma: float = 1.4, L: float = 10.0,
+                 uinf: float = 1.0, vinf: float = 1.0):
     """Isentropic vortex of strength eps at (xc, yc) in a free stream
-    rho = u = v = p = 1 on the periodic square [0, L]^2 (minimum-image distance)."""
+    rho = p = 1, (u, v) = (uinf, vinf) on the periodic square [0, L]^2
+    (minimum-image distance)."""
     h = L / n
     x = (np.arange(n) + 0.5) * h
     X, Y = np.meshgrid(x, x)
@@ -136,8 +163,8 @@ def vortex_state(n: int, eps: float, xc: float, yc: float, gamma: float = 1.4, L
     dyp = (Y - yc + 0.5 * L) % L - 0.5 * L
     r2 = dxp * dxp + dyp * dyp
     f = eps / (2.0 * math.pi) * np.exp(0.5 * (1.0 - r2))
-    u = 1.0 - f * dyp
-    v = 1.0 + f * dxp
+    u = uinf - f * dyp
+    v = vinf + f * dxp
     T = 1.0 - (gamma - 1.0) * eps * eps * np.exp(1.0 - r2) / (8.0 * gamma * math.pi ** 2)
     rho = T ** (1.0 / (gamma - 1.0))
     p = rho ** gamma
@@ -155,8 +182,8 @@ def shock_post_state(mach: float, gamma: float = 1.4):
 
 
 def shock_vortex_state(nx: int, ny: int, mach: float, xc: float, yc: float, gamma: float = 1.4,
-                       eps: float = 0.3, rc: float = 0.05, alpha: float = 0.204):
-    """Stationary normal shock at x = 0.5 on [0,2]x[0,1] with a compressible
+                       eps: float = 0.3, rc: float = 0.05, alpha: float = 0.204, xs: float = 0.5):
+    """Stationary normal shock at x = xs on [0,2]x[0,1] with a compressible
     vortex (eps, rc, alpha) advected by the upstream flow toward it."""
     h = 2.0 / nx
     x = (np.arange(nx) + 0.5) * h
@@ -172,12 +199,28 @@ def shock_vortex_state(nx: int, ny: int, mach: float, xc: float, yc: float, gamm
     v_v = -sw * (X - xc) / r
     p_v = rho_v ** gamma
     rho2, u2, p2 = shock_post_state(mach, gamma)
-    left = X < 0.5
+    left = X < xs
     rho = np.where(left, rho_v, rho2)
     u = np.where(left, u_v, u2)
     v = np.where(left, v_v, 0.0)
     p = np.where(left, p_v, p2)
     return cons_from_prim(rho, u, v, p, gamma), h, 1.0 / ny
+
+
+def ic_desc(inst) -> IcDesc:
+    """Device initial-state descriptor of an ensemble instance (the same
+    flow ``vortex_state`` / ``shock_vortex_state`` build on the host)."""
+    d = IcDesc()
+    P = inst.params
+    if inst.kind == "vortex":
+        d.kind, d.eps, d.xc, d.yc = IC_VORTEX, P["eps"], P["xc"], P["yc"]
+        d.uinf, d.vinf, d.length = P.get("uinf", 1.0), P.get("vinf", 1.0), P.get("L", 10.0)
+    elif inst.kind == "shock_vortex":
+        d.kind, d.mach, d.xc, d.yc = IC_SHOCK_VORTEX, P["mach"], P["xc"], P["yc"]
+        d.eps, d.rc, d.alpha, d.xs = P.get("eps", 0.3), P.get("rc", 0.05), P.get("alpha", 0.204), P.get("xs", 0.5)
+    else:
+        raise ValueError(f"unknown instance kind {inst.kind!r}")
+    return d
 
 
 @dataclass
@@ -189,12 +232,14 @@ class Instance:
 
 
 def make_ensemble(n: int = 256, seed: int = 2026, nmin: int = 64, nmax: int = 512, splitting: int = LLF,
-                  recon: int = WENOZ, cfl: float = 0.5, tile: int = 0):
+                  recon: int = WENOZ, cfl: float = 0.5, tile: int = 0, host_states: bool = True):
     """Seeded ensemble: half isentropic vortices (strength U[1,5], centre
     U(domain), periodic square), half shock/vortex interactions (shock Mach
     U[1.05,1.5], vortex centre U in the upstream region, extrapolation in x,
     reflective walls in y).  Grid sides are log-uniform in [nmin, nmax]
-    (shock/vortex grids are 2:1).  Returns (cfgs, states, instances)."""
+    (shock/vortex grids are 2:1).  Returns (cfgs, states, instances);
+    with host_states=False the states are None (generate them on the device
+    with ``Batch.init_ic(instances)``)."""
     rng = np.random.default_rng(seed)
     cfgs, states, insts = [], [], []
     for k in range(n):
@@ -203,7 +248,8 @@ def make_ensemble(n: int = 256, seed: int = 2026, nmin: int = 64, nmax: int = 51
         if k % 2 == 0:
             eps = rng.uniform(1.0, 5.0)
             xc, yc = rng.uniform(0.0, 10.0, size=2)
-            cons, h = vortex_state(side, eps, xc, yc)
+            h = 10.0 / side
+            cons = vortex_state(side, eps, xc, yc)[0] if host_states else None
             cfg = make_cfg(side, side, benchmark=0, splitting=splitting, recon=recon, cfl=cfl, gamma=1.4,
                            dx=h, dy=h, bc=[BC_PERIODIC] * 4, tile=tile)
             insts.append(Instance("vortex", side, side, dict(eps=eps, xc=xc, yc=yc)))
@@ -212,7 +258,8 @@ def make_ensemble(n: int = 256, seed: int = 2026, nmin: int = 64, nmax: int = 51
             xc = rng.uniform(0.15, 0.35)
             yc = rng.uniform(0.25, 0.75)
             nx, ny = side, max(8, side // 2)
-            cons, hx, hy = shock_vortex_state(nx, ny, mach, xc, yc)
+            hx, hy = 2.0 / nx, 1.0 / ny
+            cons = shock_vortex_state(nx, ny, mach, xc, yc)[0] if host_states else None
             cfg = make_cfg(nx, ny, benchmark=0, splitting=splitting, recon=recon, cfl=cfl, gamma=1.4,
                            dx=hx, dy=hy, bc=[BC_EXTRAP, BC_EXTRAP, BC_REFLECT, BC_REFLECT], tile=tile)
             insts.append(Instance("shock_vortex", nx, ny, dict(mach=mach, xc=xc, yc=yc)))

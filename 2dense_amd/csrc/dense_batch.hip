@@ -517,6 +517,135 @@ extern "C" int dense_batch_get_state(dense_batch_handle b, int i, double* cons)
     return DENSE_OK;
 }
 
+// ---------------------------------------------------------------- initial states
+// Each expression is evaluated in the order written (no contraction), so the
+// only roundings that can differ from a host evaluation of the same formulas
+// are those of exp and pow.
+
+// Remainder with the sign of the divisor (floor-modulo), exact like fmod.
+__device__ __forceinline__ double floor_mod(double a, double b)
+{
+    double m = fmod(a, b);
+    if (m != 0.0) {
+        if ((b < 0.0) != (m < 0.0)) m += b;
+    } else {
+        m = copysign(0.0, b);
+    }
+    return m;
+}
+
+__device__ __forceinline__ void cons_of(double rho, double u, double v, double p, double gamma, double* q)
+{
+    q[0] = rho;
+    q[1] = rho * u;
+    q[2] = rho * v;
+    q[3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v);
+}
+
+__device__ __forceinline__ void ic_vortex(const dense_ic& P, double gamma, double x, double y, double* q)
+{
+    constexpr double pi = 3.141592653589793;
+    const double L = P.length;
+    const double dxp = floor_mod(x - P.xc + 0.5 * L, L) - 0.5 * L;
+    const double dyp = floor_mod(y - P.yc + 0.5 * L, L) - 0.5 * L;
+    const double r2 = dxp * dxp + dyp * dyp;
+    const double f = P.eps / (2.0 * pi) * exp(0.5 * (1.0 - r2));
+    const double u = P.uinf - f * dyp;
+    const double v = P.vinf + f * dxp;
+    const double T = 1.0 - (gamma - 1.0) * P.eps * P.eps * exp(1.0 - r2) / (8.0 * gamma * (pi * pi));
+    const double rho = pow(T, 1.0 / (gamma - 1.0));
+    const double p = pow(rho, gamma);
+    cons_of(rho, u, v, p, gamma, q);
+}
+
+__device__ __forceinline__ void ic_shock_vortex(const dense_ic& P, double gamma, double x, double y, double* q)
+{
+    const double sg = sqrt(gamma);
+    const double m2 = P.mach * P.mach;
+    if (x < P.xs) {
+        const double ddx = x - P.xc, ddy = y - P.yc;
+        double r = sqrt(ddx * ddx + ddy * ddy);
+        r = r > 1e-12 ? r : 1e-12;
+        const double tau = r / P.rc;
+        const double e2 = P.eps * P.eps;
+        const double rho = pow(1.0 - (gamma - 1.0) * e2 * exp(2.0 * P.alpha * (1.0 - tau * tau)) / (4.0 * P.alpha * gamma),
+                               1.0 / (gamma - 1.0));
+        const double sw = P.eps * tau * exp(P.alpha * (1.0 - tau * tau));
+        const double u = P.mach * sg + sw * ddy / r;
+        const double v = -sw * ddx / r;
+        cons_of(rho, u, v, pow(rho, gamma), gamma, q);
+    } else {
+        // Rankine-Hugoniot state behind the shock (shock_relation.py:7-13)
+        const double rho2 = (gamma + 1.0) * m2 / (2.0 + (gamma - 1.0) * m2);
+        const double p2 = 1.0 + 2.0 * gamma / (gamma + 1.0) * (m2 - 1.0);
+        const double u2 = (2.0 + (gamma - 1.0) * m2) / ((gamma + 1.0) * m2) * (P.mach * sg);
+        cons_of(rho2, u2, 0.0, p2, gamma, q);
+    }
+}
+
+// blockIdx.y = instance; the x blocks stride over its whole plane (row pitch
+// included), so padding columns are written as zeros like dense_batch_set_state.
+__global__ void __launch_bounds__(256) batch_ic_kernel(const BatchInst* __restrict__ inst,
+                                                       const dense_ic* __restrict__ ics)
+{
+    const BatchInst& I = inst[blockIdx.y];
+    const dense_ic P = ics[blockIdx.y];
+    const long long n = I.plane;
+    double* U = I.U[0];
+    for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < n; c += (long long)gridDim.x * 256) {
+        const int j = (int)(c / I.ld), i = (int)(c - (long long)j * I.ld);
+        double q[4] = {0.0, 0.0, 0.0, 0.0};
+        if (i < I.ni) {
+            const double x = ((double)i + 0.5) * I.dx, y = ((double)j + 0.5) * I.dy;
+            if (P.kind == DENSE_IC_VORTEX)
+                ic_vortex(P, I.gamma, x, y, q);
+            else
+                ic_shock_vortex(P, I.gamma, x, y, q);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) U[k * n + c] = q[k];
+    }
+}
+
+extern "C" int dense_batch_init_ic(dense_batch_handle b, const dense_ic* ics)
+{
+    if (!b || !ics) return bfail(b, DENSE_ERR_ARG, "bad arguments");
+    long long maxplane = 0;
+    for (int i = 0; i < b->n; i++) {
+        const dense_ic& P = ics[i];
+        const double v[] = {P.eps, P.xc, P.yc, P.uinf, P.vinf, P.length, P.mach, P.xs, P.rc, P.alpha};
+        for (double x : v)
+            if (!std::isfinite(x)) return bfail(b, DENSE_ERR_ARG, "instance %d: non-finite parameter", i);
+        if (P.kind == DENSE_IC_VORTEX) {
+            if (!(P.length > 0.0)) return bfail(b, DENSE_ERR_ARG, "instance %d: vortex needs length > 0", i);
+        } else if (P.kind == DENSE_IC_SHOCK_VORTEX) {
+            if (!(P.mach >= 1.0) || !(P.rc > 0.0))
+                return bfail(b, DENSE_ERR_ARG, "instance %d: shock/vortex needs mach >= 1 and rc > 0", i);
+        } else {
+            return bfail(b, DENSE_ERR_ARG, "instance %d: unknown initial-state kind %d", i, P.kind);
+        }
+        maxplane = std::max(maxplane, b->inst[i].plane);
+    }
+    if (b->n > 65535) return bfail(b, DENSE_ERR_ARG, "at most 65535 instances");
+    dense_ic* d_ic = nullptr;
+    BCHK(b, hipStreamSynchronize(b->stream));
+    if (hipMalloc(&d_ic, sizeof(dense_ic) * b->n) != hipSuccess)
+        return bfail(b, DENSE_ERR_NOMEM, "descriptor allocation failed");
+    hipError_t e = hipMemcpy(d_ic, ics, sizeof(dense_ic) * b->n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        const long long blocks = std::min<long long>((maxplane + 255) / 256, 64);
+        hipLaunchKernelGGL(batch_ic_kernel, dim3((unsigned)blocks, (unsigned)b->n), dim3(256), 0, b->stream, b->d_inst,
+                           d_ic);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemsetAsync(b->d_ist, 0, sizeof(BatchInstState) * b->n, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(d_ic);
+    if (e != hipSuccess) return bfail(b, DENSE_ERR_HIP, "initial states: %s", hipGetErrorString(e));
+    b->need_max = true;
+    return DENSE_OK;
+}
+
 static int batch_build_graph(dense_batch* b, int nsteps)
 {
     batch_release_graph(b);

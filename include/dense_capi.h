@@ -164,6 +164,35 @@ int dense_batch_autotune(dense_batch_handle b, int trial_steps, int* tile, int* 
 int dense_batch_last_run_gpu_ms(dense_batch_handle b, double* ms);
 const char* dense_batch_last_error(dense_batch_handle b);
 
+/* Initial states generated on the device, one descriptor per instance, for the
+ * ensemble's two instance kinds.  The reference builds one state on the host
+ * (init_IVC_general.f90:13-48, init_shockvortex_general.f90:13-57); these are
+ * the same flows with the parameters free:
+ *  - DENSE_IC_VORTEX: isentropic vortex of strength eps at (xc, yc) in the
+ *    free stream rho = p = 1, (u, v) = (uinf, vinf), on the periodic square
+ *    [0, length)^2 (minimum-image distance); density T^(1/(gamma-1)),
+ *    T = 1 - (gamma-1) eps^2 e^(1-r^2) / (8 gamma pi^2), p = rho^gamma.
+ *  - DENSE_IC_SHOCK_VORTEX: normal shock at rest at x = xs, upstream
+ *    rho = p = 1, u = mach sqrt(gamma); downstream the Rankine-Hugoniot state
+ *    (shock_relation.py:7-13); upstream a compressible vortex (eps, rc, alpha)
+ *    centred at (xc, yc) as in init_shockvortex_general.f90:34-51.
+ * Cell centres are ((i + 1/2) dx, (j + 1/2) dy) with the instance's metrics.
+ * Every instance's clock is reset to 0.  Replaces dense_batch_set_state for
+ * all instances at once; no host state crosses the bus. */
+enum { DENSE_IC_VORTEX = 1, DENSE_IC_SHOCK_VORTEX = 2 };
+typedef struct dense_ic {
+    int kind;            /* DENSE_IC_VORTEX or DENSE_IC_SHOCK_VORTEX */
+    double eps;          /* vortex strength */
+    double xc, yc;       /* vortex centre */
+    double uinf, vinf;   /* free-stream velocity (vortex) */
+    double length;       /* side of the periodic square (vortex) */
+    double mach;         /* shock Mach number >= 1 (shock/vortex) */
+    double xs;           /* shock position (shock/vortex) */
+    double rc, alpha;    /* vortex core radius and decay rate (shock/vortex) */
+} dense_ic;
+
+int dense_batch_init_ic(dense_batch_handle b, const dense_ic* ics);
+
 /* ---------------------------------------------------------------------------
  * Discrete adjoint (no counterpart in the reference, which has no gradients):
  * the exact transpose of nsteps steps of Lax-Friedrichs splitting (101) +

@@ -42,3 +42,10 @@ def test_cfg_struct_layout_matches_header():
     D = dense()
     assert ctypes.sizeof(D.Cfg) == 240
     assert D.Cfg.bc_state.offset == 104 and D.Cfg.tile.offset == 232
+
+
+def test_ic_struct_layout_matches_header():
+    import importlib
+    B = importlib.import_module("2dense_amd.batch")
+    assert ctypes.sizeof(B.IcDesc) == 88
+    assert B.IcDesc.eps.offset == 8 and B.IcDesc.alpha.offset == 80

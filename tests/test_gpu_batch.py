@@ -86,3 +86,65 @@ def test_batch_rejects_shear_layer_inflow():
     with pytest.raises(D.DenseError) as e:
         B.Batch([cfg])
     assert e.value.code == -2
+
+
+def _centres_x(cfg):
+    return (np.arange(cfg.ni) + 0.5) * cfg.dx
+
+
+def test_device_initial_states_match_host_builders():
+    """The device generator evaluates the host builders' formulas in the same
+    order; only exp and pow may round differently, so every component agrees
+    to a few units in the last place of its largest value, and the
+    Rankine-Hugoniot region (no exp/pow) agrees bit for bit."""
+    B = _batch_mod()
+    cfgs, states, insts = B.make_ensemble(n=10, seed=3, nmin=16, nmax=96)
+    print(f"device ICs: sizes {[(i.ni, i.nj) for i in insts]}", flush=True)
+    with B.Batch(cfgs) as b:
+        b.init_ic(insts)
+        assert np.all(b.t == 0.0)
+        for i, inst in enumerate(insts):
+            got, ref = b.interior(i), states[i]
+            assert got.shape == ref.shape and np.all(np.isfinite(got))
+            for k in range(4):
+                err = np.max(np.abs(got[k] - ref[k])) / np.max(np.abs(ref[k]))
+                assert err <= 1e-14, f"instance {i} ({inst.kind}) component {k}: {err:.3e}"
+            if inst.kind == "shock_vortex":
+                right = _centres_x(cfgs[i]) >= 0.5
+                assert right.any()
+                assert np.array_equal(got[:, :, right], ref[:, :, right])
+
+
+def test_device_initial_states_evolve_like_host_states():
+    B = _batch_mod()
+    cfgs, states, insts = B.make_ensemble(n=6, seed=9, nmin=16, nmax=64)
+    steps = 30
+    with B.Batch(cfgs) as dev, B.Batch(cfgs) as host:
+        dev.init_ic(insts)
+        for i, c in enumerate(states):
+            host.set_state(i, c)
+        hd = dev.run(steps, want_dts=True)
+        hh = host.run(steps, want_dts=True)
+        assert np.allclose(hd, hh, rtol=1e-12, atol=0)
+        for i in range(len(cfgs)):
+            a, r = dev.interior(i), host.interior(i)
+            err = np.max(np.abs(a - r)) / np.max(np.abs(r))
+            assert err <= 1e-11, f"instance {i} ({insts[i].kind}): {err:.3e}"
+
+
+def test_device_initial_states_reject_bad_descriptors():
+    D = dense()
+    B = _batch_mod()
+    cfgs, _, insts = B.make_ensemble(n=2, seed=1, nmin=16, nmax=24, host_states=False)
+    with B.Batch(cfgs) as b:
+        good = [B.ic_desc(x) for x in insts]
+        for field, value in (("kind", 3), ("mach", 0.5), ("rc", 0.0), ("eps", float("nan"))):
+            bad = [B.ic_desc(x) for x in insts]
+            k = 1 if field in ("mach", "rc") else 0
+            setattr(bad[k], field, value)
+            with pytest.raises(D.DenseError) as e:
+                b.init_ic(bad)
+            assert e.value.code == -1
+        b.init_ic(good)
+        b.run(2)
+        assert np.all(np.isfinite(b.interior(0)))

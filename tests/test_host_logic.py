@@ -145,6 +145,25 @@ def test_make_ensemble_is_seeded_and_in_range():
         assert 16 <= cfg.ni <= 64
 
 
+def test_ensemble_descriptors_follow_instances():
+    B = importlib.import_module("2dense_amd.batch")
+    c1, s1, i1 = B.make_ensemble(n=6, seed=4, nmin=16, nmax=48)
+    c2, s2, i2 = B.make_ensemble(n=6, seed=4, nmin=16, nmax=48, host_states=False)
+    assert all(s is None for s in s2)
+    assert [(a.ni, a.nj, a.dx, a.dy) for a in c1] == [(a.ni, a.nj, a.dx, a.dy) for a in c2]
+    for inst in i2:
+        d = B.ic_desc(inst)
+        if inst.kind == "vortex":
+            assert d.kind == B.IC_VORTEX and (d.eps, d.xc, d.yc) == (inst.params["eps"], inst.params["xc"],
+                                                                    inst.params["yc"])
+            assert (d.uinf, d.vinf, d.length) == (1.0, 1.0, 10.0)
+        else:
+            assert d.kind == B.IC_SHOCK_VORTEX and d.mach == inst.params["mach"]
+            assert (d.eps, d.rc, d.alpha, d.xs) == (0.3, 0.05, 0.204, 0.5)
+    with pytest.raises(ValueError):
+        B.ic_desc(B.Instance("blast", 8, 8, {}))
+
+
 def test_fourier_ic_base_state_and_linearity():
     import torch
     A = importlib.import_module("2dense_amd.adjoint")
