@@ -159,3 +159,57 @@ def test_batched_full_ensemble_samples_equal_single_runs():
             got = b.interior(i)
             assert np.all(np.isfinite(got))
             assert np.array_equal(got, ref), f"instance {i} ({insts[i].kind} {insts[i].ni}x{insts[i].nj})"
+
+
+def _rt_reflecting_cfg(p):
+    """Rayleigh-Taylor as the workload states it, with reflecting side walls:
+    the reference's own RT boundary is periodic in x (boundary_rt_general.f90:
+    129-151), so this variant goes through benchmark 0 with explicit kinds."""
+    D = dense()
+    top_bottom = [[0.0] * 4, [0.0] * 4, [2.0, 0.0, 0.0, 1.0], [1.0, 0.0, 0.0, 2.5]]
+    return D.make_cfg(p.ni, p.nj, benchmark=0, splitting=p.splitting, recon=p.recon, cfl=p.cfl, gamma=p.gamma,
+                      if_bodyforce=True, dx=p.dx, dy=p.dy,
+                      bc=[D.BC_REFLECT, D.BC_REFLECT, D.BC_DIRICHLET, D.BC_DIRICHLET], bc_state=top_bottom)
+
+
+def _mirror_defect(u):
+    sgn = np.array([1.0, -1.0, 1.0, 1.0])[:, None, None]
+    return np.max(np.abs(u - sgn * u[:, :, ::-1])) / np.max(np.abs(u))
+
+
+def test_rt_reflecting_sides_256x1024_vs_oracle(oracle):
+    """Against the C oracle with the same boundary kinds (its x-side reflection
+    is pinned by the reference's Sedov fixtures), bit for bit, plus the mirror
+    symmetry about x = 1/8 that reflecting walls keep."""
+    ni, nj, steps = 256, 1024, 20
+    p = oracle.problem_params(3, ni, nj, splitting=105, recon=255, cfl=0.5, nthreads=_threads())
+    p.bc[0] = p.bc[1] = oracle.BC_REFLECT
+    o = oracle.Oracle(p)
+    o.init_state(3)
+    ic = o.cons()
+    ref_dts = []
+    for _ in range(steps):
+        o.step(1)
+        ref_dts.append(o.dt)
+    ref = o.interior()
+    o.close()
+    print(f"RT reflecting sides {ni}x{nj} 105, {steps} steps", flush=True)
+    got, dts = _gpu_run(_rt_reflecting_cfg(p), ic, steps)
+    assert np.array_equal(got, ref)
+    assert np.array_equal(dts, np.array(ref_dts))
+    assert _mirror_defect(got) < 1e-13
+
+
+def test_rt_reflecting_sides_1024x4096_mirror_symmetry(oracle):
+    ni, nj, steps = 1024, 4096, 40
+    p = oracle.problem_params(3, ni, nj, splitting=105, recon=255, cfl=0.5)
+    o = oracle.Oracle(p)
+    o.init_state(3)
+    ic = o.cons()
+    o.close()
+    print(f"RT reflecting sides {ni}x{nj} 105, {steps} steps", flush=True)
+    got, _ = _gpu_run(_rt_reflecting_cfg(p), ic, steps)
+    assert np.all(np.isfinite(got))
+    d = _mirror_defect(got)
+    print(f"mirror symmetry defect {d:.3e}", flush=True)
+    assert d < 1e-12
