@@ -250,18 +250,31 @@ def read_dat2d(path: str) -> dict:
     return dict(step=step, time=time, ni=ni, nj=nj, cons=cons, x=x, y=y, dx=dx, dy=dy)
 
 
-def run_reference(*, steps: int, binary: str = REF_BIN, workdir: str | None = None, **nml) -> dict:
+def run_reference(*, steps: int, binary: str = REF_BIN, workdir: str | None = None,
+                  until_time: float | None = None, **nml) -> dict:
     """Run the reference for `steps` steps and return its .dat2d dump at that step,
-    plus its stdout (per-step dt)."""
+    plus its stdout (per-step dt).  With until_time, run while t < until_time
+    (at most `steps` steps) instead: DT_SAVE = TOTAL_TIME makes the reference's
+    save clock (elasp_time, general_main.f90:38) reach it at the same step as
+    the flow clock, so it dumps exactly the last step; that dump is returned."""
     if not os.path.exists(binary):
         raise FileNotFoundError(f"reference binary not built: {binary} (run oracle/build_ref.sh)")
     own = workdir is None
     wd = tempfile.mkdtemp(prefix="dense_ref_") if own else workdir
     try:
-        write_input_in(os.path.join(wd, "input.in"), total_step=steps, save_step=steps, **nml)
+        if until_time is None:
+            write_input_in(os.path.join(wd, "input.in"), total_step=steps, save_step=steps, **nml)
+        else:
+            write_input_in(os.path.join(wd, "input.in"), total_step=steps, save_step=steps + 1,
+                           total_time=until_time, dt_save=until_time, **nml)
         env = dict(os.environ)
         out = subprocess.run([binary], cwd=wd, capture_output=True, text=True, env=env, check=True)
-        d = read_dat2d(os.path.join(wd, f"2d_{steps}.dat2d"))
+        if until_time is not None:
+            dumps = [f for f in os.listdir(wd) if f.startswith("2d_") and f.endswith(".dat2d")]
+            last = max(int(f[3:-6]) for f in dumps)
+            d = read_dat2d(os.path.join(wd, f"2d_{last}.dat2d"))
+        else:
+            d = read_dat2d(os.path.join(wd, f"2d_{steps}.dat2d"))
         dts = []
         for line in out.stdout.splitlines():
             if "Flow time=" in line and "dt=" in line:

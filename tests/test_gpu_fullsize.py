@@ -213,3 +213,34 @@ def test_rt_reflecting_sides_1024x4096_mirror_symmetry(oracle):
     d = _mirror_defect(got)
     print(f"mirror symmetry defect {d:.3e}", flush=True)
     assert d < 1e-12
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.skipif(os.environ.get("DENSE_LONG") != "1",
+                    reason="full run to t = 0.2 (about 1,650 steps; the reference takes minutes on the host cores); "
+                           "set DENSE_LONG=1")
+def test_dmr_960x240_to_t_end_vs_reference(oracle):
+    """The double Mach workload as configured, run to t = 0.2 with the
+    reference's clock semantics (no clipping of the last step), against the
+    reference binary: same number of steps, same final time, same state."""
+    ni, nj, t_end = 960, 240, 0.2
+    binary = _ref_binary(oracle, 104)
+    print(f"reference DMR {ni}x{nj} 104 to t = {t_end}", flush=True)
+    d = oracle.run_reference(steps=100000, until_time=t_end, binary=binary, benchmark=5, ni=ni, nj=nj,
+                             splitting=104, recon=255, cfl=0.6, threads=_threads())
+    p = oracle.problem_params(5, ni, nj, splitting=104, recon=255, cfl=0.6)
+    o = oracle.Oracle(p)
+    o.init_state(5)
+    ic = o.cons()
+    o.close()
+    D = dense()
+    with D.Solver(D.make_cfg(ni, nj, benchmark=5, splitting=104, recon=255, cfl=0.6)) as s:
+        s.set_state(ic)
+        n = s.run(100000, total_time=t_end)
+        got, t, step = s.interior(), s.current_time, s.current_step
+    ref = d["cons"][:, 4:-4, 4:-4]
+    print(f"GPU {n} steps to t = {t!r}; reference {d['step']} steps to t = {d['time']!r}; "
+          f"density L1 {_l1_density(got, ref, d['dx'], d['dy']):.3e}", flush=True)
+    assert step == d["step"] and t == d["time"]
+    assert len(d["dts"]) == step
+    assert np.array_equal(got, ref)
