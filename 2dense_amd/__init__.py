@@ -119,12 +119,15 @@ def make_cfg(ni: int, nj: int, *, benchmark: int = 0, splitting: int = LLF, reco
              stages: int = 3, if_const_dt: bool = False, if_bodyforce: bool | None = None,
              dx: float | None = None, dy: float | None = None, bc=None, bc_state=None,
              tile: int = 0) -> Cfg:
-    """Build a dense_cfg.  For benchmark 1/2/3/4/5/9 the grid metrics and boundary
+    """Build a dense_cfg.  For benchmark 1/2/3/4/5/6/9 the grid metrics and boundary
     rules come from the reference's canned problem (dense_problem_defaults;
     benchmark 3 also sets gamma = 5/3 and the gravity body force, as
     init_RT_general.f90:9-10 does); for benchmark 0 give dx, dy, bc (4 kinds)
     and bc_state (4 x (rho,u,v,p)).  gamma=None keeps the problem's value (1.4
-    unless the problem sets one)."""
+    unless the problem sets one).  tile: 0 lets the library pick the tile shape
+    (32x15 for splittings 101/102/105 on grids of 80,000 cells or more, else
+    16x15), 1-3 force 32x7, 8x7, 32x15 and 4 forces 16x15; results do not
+    depend on it."""
     lib = load_library()
     c = Cfg()
     c.gamma = 1.4

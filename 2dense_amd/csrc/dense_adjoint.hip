@@ -277,7 +277,7 @@ struct dense_adjoint {
     int scheme = -1, nsteps = 0, k = 0, nseg = 0;
     hipStream_t stream = nullptr;   // forward pass and the reverse sweep
     hipStream_t rstream = nullptr;  // recomputation of the next segment, overlapped with the reverse sweep
-    int ld = 0, tiles_x = 0, ntiles = 0;
+    int ld = 0, tile = 0, tiles_x = 0, ntiles = 0;
     long long plane = 0;
     DevState* st = nullptr;
     double* dt_hist = nullptr;
@@ -357,7 +357,7 @@ static int adj_enqueue_step(dense_adjoint* a, hipStream_t strm, double* U0, doub
         s.un = U0;
         s.us = ins[k - 1];
         s.out = outs[k - 1];
-        ACHK(a, launch_stage(a->scheme, a->cfg.tile, s, strm));
+        ACHK(a, launch_stage(a->scheme, a->tile, s, strm));
     }
     return DENSE_OK;
 }
@@ -424,7 +424,8 @@ extern "C" int dense_adj_create(const dense_cfg* cfg, int nsteps, int checkpoint
     a->nseg = (nsteps + a->k - 1) / a->k;
     a->ld = (c.ni + 31) / 32 * 32;
     a->plane = (long long)a->ld * c.nj;
-    a->ntiles = tile_count(c.tile, c.ni, c.nj, &a->tiles_x);
+    a->tile = resolve_tile(c.tile, c.splitting, (long long)c.ni * c.nj);
+    a->ntiles = tile_count(a->tile, c.ni, c.nj, &a->tiles_x);
     const size_t st_bytes = sizeof(double) * 4 * a->plane;
     const size_t nf = (size_t)c.ni * c.nj;
     if (hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking) != hipSuccess ||

@@ -410,7 +410,7 @@ extern "C" int dense_batch_create(const dense_cfg* cfgs, int n, dense_batch_hand
         }
     }
     if (b->stages < 2) return bail(bfail(nullptr, DENSE_ERR_UNSUPPORTED, "batched mode needs stages >= 2"));
-    b->tile = cfgs[0].tile;
+    b->tile = (cfgs[0].tile == 4) ? 0 : cfgs[0].tile;  // the batch's own schedule: 0 is the 16x15 shape
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
         return bail(bfail(nullptr, DENSE_ERR_HIP, "stream/event creation failed"));

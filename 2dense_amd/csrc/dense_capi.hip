@@ -31,6 +31,7 @@ struct dense_solver {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;  // bracket the last dense_run on the stream
     double last_run_ms = 0.0;
     int last_run_launches = 0;
+    int tile = 0;            // resolved tile shape (resolve_tile of cfg.tile)
     int tiles_x = 0, ntiles = 0;
     double* bc_rows = nullptr;    // device: shear-layer inflow tables (2 nj), else null
     double* bct_host = nullptr;   // pinned: clock terms of the shear-layer inflow
@@ -77,7 +78,7 @@ int validate_cfg(const dense_cfg* c, char* why, size_t n)
     if (scheme_of(c->splitting, c->recon) < 0)
         return snprintf(why, n, "unsupported switch pair splitting=%d recon=%d", c->splitting, c->recon),
                DENSE_ERR_UNSUPPORTED;
-    if (c->tile < 0 || c->tile > 3) return snprintf(why, n, "tile index out of range"), DENSE_ERR_ARG;
+    if (c->tile < 0 || c->tile > 4) return snprintf(why, n, "tile index out of range"), DENSE_ERR_ARG;
     for (int s = 0; s < 4; s++)
         if (c->bc[s] < 0 || c->bc[s] > 6) return snprintf(why, n, "bad boundary kind on side %d", s), DENSE_ERR_ARG;
     for (int s = 1; s < 4; s++)
@@ -212,7 +213,7 @@ static int enqueue_step(dense_solver* h)
         a.un = h->U[0];
         a.us = s == 1 ? h->U[0] : h->U[s - 1];
         a.out = (s == S && S > 1) ? h->U[0] : h->U[s];
-        HIPCHK(h, launch_stage(h->scheme, h->cfg.tile, a, h->stream));
+        HIPCHK(h, launch_stage(h->scheme, h->tile, a, h->stream));
     }
     if (S == 1)
         HIPCHK(h, hipMemcpyAsync(h->U[0], h->U[1], sizeof(double) * 4 * h->plane, hipMemcpyDeviceToDevice, h->stream));
@@ -252,7 +253,8 @@ extern "C" int dense_create(const dense_cfg* cfg, dense_handle* out)
     h->scheme = scheme_of(c.splitting, c.recon);
     h->ld = (c.ni + 31) / 32 * 32;
     h->plane = (long long)h->ld * c.nj;
-    h->ntiles = tile_count(c.tile, c.ni, c.nj, &h->tiles_x);
+    h->tile = resolve_tile(c.tile, c.splitting, (long long)c.ni * c.nj);
+    h->ntiles = tile_count(h->tile, c.ni, c.nj, &h->tiles_x);
     auto bail = [&](int code) {
         dense_destroy(h);
         return code;
@@ -535,6 +537,8 @@ extern "C" int dense_update_params(dense_handle h, const dense_cfg* cfg)
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->cfg = c;
     h->scheme = scheme_of(c.splitting, c.recon);
+    h->tile = resolve_tile(c.tile, c.splitting, (long long)c.ni * c.nj);  // the automatic shape follows the splitting
+    h->ntiles = tile_count(h->tile, c.ni, c.nj, &h->tiles_x);
     release_graph(h);  // kernel arguments changed
     // the wave speeds of the current state depend on gamma
     int r2 = read_state(h);

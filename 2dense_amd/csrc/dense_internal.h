@@ -116,4 +116,21 @@ hipError_t launch_set_clock(DevState* st, double t, double t_end, int step, int 
 hipError_t launch_fill_padded(const StageArgs& a, double* cons_pad, double* prim_pad, hipStream_t s);
 int tile_count(int tile, int ni, int nj, int* tiles_x);
 
+// dense_cfg.tile -> index of the tile shape list (dense_tile.h: 0 16x15,
+// 1 32x7, 2 8x7, 3 32x15).  0 chooses by splitting and grid size,
+// from timings on MI355X: the 32x15 tile (fewer halo cells and duplicated
+// faces per cell) is 5-110 % faster for the component-wise Lax-Friedrichs
+// paths (101, 102) and AdaWENO (105) from about 80,000 cells up; the 16x15
+// tile is 8-13 % faster there for Steger-Warming (103) and the characteristic
+// scheme (104), and is the faster shape for every path on smaller grids, where
+// more, smaller workgroups fill the CUs.  1..3 force those shapes and 4 forces
+// the 16x15 shape.
+inline int resolve_tile(int tile, int splitting, long long cells)
+{
+    if (tile == 4) return 0;
+    if (tile != 0) return tile;
+    const bool lf = splitting == 101 || splitting == 102 || splitting == 105;
+    return (lf && cells >= 80000) ? 3 : 0;
+}
+
 }  // namespace dense

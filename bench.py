@@ -83,7 +83,7 @@ def dmr_initial_state(ni: int, nj: int, gamma: float = 1.4) -> np.ndarray:
 
 def gpu_leg(steps: int, warmup: int):
     D = importlib.import_module("2dense_amd")
-    cfg = D.make_cfg(NI, NJ, benchmark=5, splitting=SPLITTING, recon=RECON, cfl=CFL, tile=0)
+    cfg = D.make_cfg(NI, NJ, benchmark=5, splitting=SPLITTING, recon=RECON, cfl=CFL, tile=4)  # 16x15
     s = D.Solver(cfg)
     s.set_state(dmr_initial_state(NI, NJ))
     if warmup:
