@@ -25,8 +25,12 @@ bracketed by device synchronisation.
   adjoint: gradient of a density mismatch after 200 steps (256^2 periodic
             vortex, 64 Fourier modes, LLF + WENO-Z, checkpoint every 20 steps)
             vs a plain forward solve of the same problem.
+  other_workloads: GPU cell-updates/s of the other configured workloads
+            (isentropic vortex 200^2, double Mach 960x240, Rayleigh-Taylor
+            256x1024 and 1024x4096); reported beside the metric, not in it.
 
     python bench.py [--gpus 1] [--steps 640] [--warmup 32] [--no-cpu] [--no-batched] [--no-adjoint]
+                    [--no-workloads]
 
 Steps are replayed in graph chunks of 16; multiples of 16 keep every
 recorded stage launch a working one.
@@ -79,6 +83,66 @@ def dmr_initial_state(ni: int, nj: int, gamma: float = 1.4) -> np.ndarray:
     cons[2] = rho * v
     cons[3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
     return cons
+
+
+def rt_initial_state(ni: int, nj: int, gamma: float = 5.0 / 3.0) -> np.ndarray:
+    """Interior conserved state of the Rayleigh-Taylor problem
+    (init_RT_general.f90:16-48): node-centred grid on [0,0.25]x[0,1], heavy
+    fluid rho = 2 below y = 0.5 with p = 2y + 1, light fluid rho = 1 above with
+    p = y + 1.5, v = -0.025 c cos(8 pi x) (the light side's literal is single
+    precision, :47)."""
+    dx, dy = 0.25 / (ni - 1.0), 1.0 / (nj - 1.0)
+    x = np.arange(ni) * dx
+    y = np.arange(nj) * dy
+    X, Y = np.meshgrid(x, y)
+    heavy = Y <= 0.5
+    rho = np.where(heavy, 2.0, 1.0)
+    p = np.where(heavy, 2.0 * Y + 1.0, Y + 1.5)
+    c = np.sqrt(gamma * p / rho)
+    amp = np.where(heavy, -0.025, float(np.float32(-0.025)))
+    v = amp * c * np.cos(8.0 * np.pi * X)
+    return np.stack([rho, np.zeros_like(rho), rho * v, p / (gamma - 1.0) + 0.5 * rho * v * v])
+
+
+def workloads_leg(reps: int = 3):
+    """GPU throughput of the other configured workloads (parity cases, not the
+    metric): the first run captures the graphs, then `reps` timed runs of the
+    same steps from the same initial state; best wall time, cell-updates/s."""
+    D = importlib.import_module("2dense_amd")
+    B = importlib.import_module("2dense_amd.batch")
+    vort, h = B.vortex_state(200, 5.0, 5.0, 5.0)
+    cases = [
+        ("isentropic vortex 200x200, LLF + WENO-JS (101/253), CFL 0.5, periodic",
+         D.make_cfg(200, 200, benchmark=0, splitting=101, recon=253, cfl=0.5, gamma=1.4, dx=h, dy=h,
+                    bc=[D.BC_PERIODIC] * 4), vort, 100),
+        ("double Mach reflection 960x240, characteristic WENO-Z + GLF (104/255), CFL 0.6",
+         D.make_cfg(960, 240, benchmark=5, splitting=104, recon=255, cfl=0.6), dmr_initial_state(960, 240), 320),
+        ("Rayleigh-Taylor 256x1024, AdaWENO (105/255), gravity, CFL 0.5",
+         D.make_cfg(256, 1024, benchmark=3, splitting=105, recon=255, cfl=0.5), rt_initial_state(256, 1024), 320),
+        ("Rayleigh-Taylor 1024x4096, AdaWENO (105/255), gravity, CFL 0.5",
+         D.make_cfg(1024, 4096, benchmark=3, splitting=105, recon=255, cfl=0.5), rt_initial_state(1024, 4096), 64),
+    ]
+    out = []
+    for name, cfg, cons, steps in cases:
+        best = None
+        with D.Solver(cfg) as s:
+            for r in range(reps + 1):
+                s.set_state(cons)
+                s.current_time, s.current_step = 0.0, 0
+                s.sync()
+                t0 = time.perf_counter()
+                n = s.run(steps)
+                s.sync()
+                wall = time.perf_counter() - t0
+                if n != steps:
+                    raise RuntimeError(f"{name}: ran {n} of {steps} steps")
+                if r > 0 and (best is None or wall < best[0]):
+                    best = (wall, s.last_run_gpu_ms()[0])
+        cells = cfg.ni * cfg.nj
+        out.append(dict(workload=name, ni=cfg.ni, nj=cfg.nj, steps=steps,
+                        cell_updates_per_s=cells * steps / best[0], gpu_cell_updates_per_s=cells * steps / (best[1] * 1e-3),
+                        ms_per_step=best[0] / steps * 1e3))
+    return out
 
 
 def gpu_leg(steps: int, warmup: int):
@@ -282,6 +346,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-workloads", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run one plain device-to-device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -311,6 +376,9 @@ def main():
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_leg()
+    workloads = None
+    if not args.no_workloads:
+        workloads = workloads_leg()
     cpu = None
     if not args.no_cpu:
         try:
@@ -337,6 +405,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adjoint,
+        "other_workloads": workloads,
     }
     if cpu and cpu.get("value"):
         out["speedup_vs_cpu"] = value / cpu["value"]

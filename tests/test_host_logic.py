@@ -185,3 +185,19 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=300)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+@pytest.mark.parametrize("benchmark,ni,nj", [(5, 96, 24), (5, 120, 30), (3, 32, 128), (3, 64, 256)])
+def test_bench_initial_states_equal_the_oracle(oracle, benchmark, ni, nj):
+    """bench.py builds its double Mach and Rayleigh-Taylor initial states
+    itself; they equal the oracle's restatement of init_doublemach_general.f90
+    and init_RT_general.f90 bit for bit."""
+    sys.path.insert(0, ROOT)
+    import bench
+    p = oracle.problem_params(benchmark, ni, nj)
+    o = oracle.Oracle(p)
+    o.init_state(benchmark)
+    ref = o.interior()
+    o.close()
+    got = bench.dmr_initial_state(ni, nj) if benchmark == 5 else bench.rt_initial_state(ni, nj)
+    assert np.array_equal(got, ref)
