@@ -5,6 +5,7 @@
 // clocks, so every instance advances with its own CFL step exactly as it
 // would alone (bit for bit: the per-tile work is the single-block code).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -173,7 +174,11 @@ __global__ void batch_set_run_kernel(BatchState* bs, int nsteps)
 template <int SCHEME, class T>
 static hipError_t launch_batch_one(int grid_pct, const BatchArgs& b, hipStream_t s)
 {
-    static int capacity = 0;
+    // resident blocks of this kernel on the device, computed once (several
+    // handles may launch from different host threads: relaxed atomics, every
+    // writer stores the same value)
+    static std::atomic<int> cached{0};
+    int capacity = cached.load(std::memory_order_relaxed);
     if (capacity == 0) {
         int dev = 0, per_cu = 0, cus = 0;
         hipError_t e = hipGetDevice(&dev);
@@ -182,6 +187,7 @@ static hipError_t launch_batch_one(int grid_pct, const BatchArgs& b, hipStream_t
             e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_stage_kernel<SCHEME, T>, T::NT, 0);
         if (e != hipSuccess) return e;
         capacity = (per_cu < 1 ? 1 : per_cu) * (cus < 1 ? 1 : cus);
+        cached.store(capacity, std::memory_order_relaxed);
     }
     int grid = (int)((long long)capacity * grid_pct / 100) / 8 * 8;
     const int need = (b.ntiles + 7) / 8 * 8;

@@ -474,7 +474,8 @@ extern "C" int dense_run(dense_handle h, int max_steps, double total_time, int t
         if (rc) return rc;
     }
     // the device stops itself once the clock passes total_time/total_step
-    int step_end = *step + max_steps < total_step ? *step + max_steps : total_step;
+    const long long want = (long long)*step + max_steps;  // no int overflow near INT_MAX
+    const int step_end = want < total_step ? (int)want : total_step;
     HIPCHK(h, launch_set_clock(h->st, *t, total_time, *step, step_end, h->stream));
     const int launches = (max_steps + chunk - 1) / chunk;
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));

@@ -1,5 +1,7 @@
 // Fused RK-stage kernel of the explicit update (one block) and its helpers
 // (gfx950).  The per-tile work lives in dense_tile.h.
+#include <atomic>
+
 #include "dense_tile.h"
 
 namespace dense {
@@ -110,7 +112,11 @@ __global__ void __launch_bounds__(256) fill_padded_kernel(StageArgs a, double* c
 template <int SCHEME, class T>
 static hipError_t launch_one(const StageArgs& a, hipStream_t s)
 {
-    static int capacity = 0;
+    // resident blocks of this kernel on the device, computed once (several
+    // handles may launch from different host threads: relaxed atomics, every
+    // writer stores the same value)
+    static std::atomic<int> cached{0};
+    int capacity = cached.load(std::memory_order_relaxed);
     if (capacity == 0) {
         int dev = 0, per_cu = 0, cus = 0;
         hipError_t e = hipGetDevice(&dev);
@@ -119,6 +125,7 @@ static hipError_t launch_one(const StageArgs& a, hipStream_t s)
             e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, stage_kernel<SCHEME, T>, T::NT, 0);
         if (e != hipSuccess) return e;
         capacity = (per_cu < 1 ? 1 : per_cu) * (cus < 1 ? 1 : cus);
+        cached.store(capacity, std::memory_order_relaxed);
     }
     int grid = (a.ntiles + 7) / 8 * 8;
     if (grid > capacity) grid = capacity / 8 * 8;

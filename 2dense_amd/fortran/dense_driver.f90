@@ -4,11 +4,10 @@
 ! names and switch codes, flow_mod_general.f90:51-180, read in the order of
 ! read_par_namelist, :313-332), builds one of the canned problems
 ! (benchmark 1 isentropic vortex, 2 Sedov blast, 3 Rayleigh-Taylor,
-! 4 Richtmyer-Meshkov, 5 double Mach reflection, 6 shear layer, 9
-! shock/vortex), advances
-! it on the GPU through the C ABI and
-! writes the reference's outputs: per-step stdout, 2d_<step>.dat2d binary
-! dumps, 2d_<step>.dat Tecplot files and CpuTime.dat (general_main.f90:1-63).
+! 4 Richtmyer-Meshkov, 5 double Mach reflection, 6 shear layer,
+! 9 shock/vortex), advances it on the GPU through the C ABI and writes the
+! reference's outputs: per-step stdout, 2d_<step>.dat2d binary dumps,
+! 2d_<step>.dat Tecplot files and CpuTime.dat (general_main.f90:1-63).
 !
 ! Addition over the reference: START_STEP in TIME_STEP_PARS restarts from
 ! 2d_<START_STEP>.dat2d (the reference's loader, read_WriteAndLoad.f90:30-70,
