@@ -54,7 +54,25 @@ static const double a322 = -1.0 / 6.0;
 static const double c30 = 0.1;
 static const double c31 = 0.6;
 static const double c32 = 0.3;
-#define WENO_SS SP(1.0e-6) /* weno_mod.f90:37, single literal */
+/* Planted mistakes for the tests that show the reference's fixtures catch
+ * them (tests/test_oracle_mutations.py); 0 = the restatement as written.
+ * Each value changes one formula the way a plausible slip would. */
+static int g_mut = 0;
+static int s_cur_recon = 255; /* reconstruction switch of the block being advanced (mutation 8) */
+enum {
+    MUT_WENO_EPS_DOUBLE = 1,  /* WENO epsilon as the double 1e-6, not the widened single literal */
+    MUT_BODYFORCE_23 = 2,     /* stage-3 body-force coefficient 2/3 in double precision */
+    MUT_LLF_GLOBAL = 3,       /* LLF x fluxes with the global wave speed (that is GLF) */
+    MUT_DMR_TOP_T_NEXT = 4,   /* double-Mach top boundary at t^n + dt instead of t^n */
+    MUT_EXTRAP_ORDER = 5,     /* extrapolation evaluated as (4 q1 - q2)/3 */
+    MUT_ROE_SWAP = 6,         /* Roe average velocity weights swapped between the two cells */
+    MUT_RK3_WEIGHT = 7,       /* stage-3 weight 2/3 formed as 1 - 1/3 (one ulp off) */
+    MUT_CH_HONOURS_RECON = 8, /* characteristic schemes use WENO-JS unless the switch is 255 */
+    MUT_DT_ASSOC = 9          /* CFL step as cfl*(dt_xi*dt_eta)/(dt_xi+dt_eta) */
+};
+void ora_set_mutation(int m) { g_mut = m; }
+#define WENO_SS (g_mut == MUT_WENO_EPS_DOUBLE ? 1.0e-6 : SP(1.0e-6)) /* weno_mod.f90:37, single literal */
+#define EXTRAP(q1, q2) (g_mut == MUT_EXTRAP_ORDER ? (4.0 * (q1) - (q2)) / 3.0 : (q1) * 4.0 / 3.0 - (q2) / 3.0)
 
 struct ora_block {
     ora_params p;
@@ -293,7 +311,7 @@ static inline void roe_average(double gamma, double rhol, double ul, double vl, 
     double hl = pl / (gamma - 1) / rhol + 0.5 * (ul * ul + vl * vl) + pl / rhol;
     double hr = pr / (gamma - 1) / rhor + 0.5 * (ur * ur + vr * vr) + pr / rhor;
     double sr = sqrt(rhor), sl = sqrt(rhol);
-    *u0 = (sr * ur + sl * ul) / (sr + sl);
+    *u0 = (g_mut == MUT_ROE_SWAP ? sl * ur + sr * ul : sr * ur + sl * ul) / (sr + sl);
     *v0 = (sr * vr + sl * vl) / (sr + sl);
     *h0 = (sr * hr + sl * hl) / (sr + sl);
     *alpha0 = sqrt((gamma - 1.0) * (*h0 - 0.5 * (*u0 * *u0 + *v0 * *v0)));
@@ -406,11 +424,11 @@ static void boundary(ora_block* b)
         for (k = 1; k <= NEQ; k++)
             for (j = BF + 1; j <= nj + BF; j++)
                 for (i = BF; i >= 1; i--)
-                    CONS(b, i, j, k) = CONS(b, i + 1, j, k) * 4.0 / 3.0 - CONS(b, i + 2, j, k) / 3.0;
+                    CONS(b, i, j, k) = EXTRAP(CONS(b, i + 1, j, k), CONS(b, i + 2, j, k));
         for (k = 1; k <= NPRIM; k++)
             for (j = BF + 1; j <= nj + BF; j++)
                 for (i = BF; i >= 1; i--)
-                    PRIM(b, i, j, k) = PRIM(b, i + 1, j, k) * 4.0 / 3.0 - PRIM(b, i + 2, j, k) / 3.0;
+                    PRIM(b, i, j, k) = EXTRAP(PRIM(b, i + 1, j, k), PRIM(b, i + 2, j, k));
         break;
     case ORA_BC_REFLECT:
         for (j = BF + 1; j <= nj + BF; j++)
@@ -458,11 +476,11 @@ static void boundary(ora_block* b)
         for (k = 1; k <= NEQ; k++)
             for (j = BF + 1; j <= nj + BF; j++)
                 for (i = ni + BF + 1; i <= ni + 2 * BF; i++)
-                    CONS(b, i, j, k) = CONS(b, i - 1, j, k) * 4.0 / 3.0 - CONS(b, i - 2, j, k) / 3.0;
+                    CONS(b, i, j, k) = EXTRAP(CONS(b, i - 1, j, k), CONS(b, i - 2, j, k));
         for (k = 1; k <= NPRIM; k++)
             for (j = BF + 1; j <= nj + BF; j++)
                 for (i = ni + BF + 1; i <= ni + 2 * BF; i++)
-                    PRIM(b, i, j, k) = PRIM(b, i - 1, j, k) * 4.0 / 3.0 - PRIM(b, i - 2, j, k) / 3.0;
+                    PRIM(b, i, j, k) = EXTRAP(PRIM(b, i - 1, j, k), PRIM(b, i - 2, j, k));
         break;
     case ORA_BC_REFLECT:
         for (j = BF + 1; j <= nj + BF; j++)
@@ -489,11 +507,11 @@ static void boundary(ora_block* b)
         for (k = 1; k <= NEQ; k++)
             for (i = BF + 1; i <= ni + BF; i++)
                 for (j = BF; j >= 1; j--)
-                    CONS(b, i, j, k) = CONS(b, i, j + 1, k) * 4.0 / 3.0 - CONS(b, i, j + 2, k) / 3.0;
+                    CONS(b, i, j, k) = EXTRAP(CONS(b, i, j + 1, k), CONS(b, i, j + 2, k));
         for (k = 1; k <= NPRIM; k++)
             for (i = BF + 1; i <= ni + BF; i++)
                 for (j = BF; j >= 1; j--)
-                    PRIM(b, i, j, k) = PRIM(b, i, j + 1, k) * 4.0 / 3.0 - PRIM(b, i, j + 2, k) / 3.0;
+                    PRIM(b, i, j, k) = EXTRAP(PRIM(b, i, j + 1, k), PRIM(b, i, j + 2, k));
         break;
     case ORA_BC_REFLECT:
         for (j = 1; j <= BF; j++)
@@ -531,11 +549,11 @@ static void boundary(ora_block* b)
         for (k = 1; k <= NEQ; k++)
             for (i = BF + 1; i <= ni + BF; i++)
                 for (j = nj + BF + 1; j <= nj + 2 * BF; j++)
-                    CONS(b, i, j, k) = CONS(b, i, j - 1, k) * 4.0 / 3.0 - CONS(b, i, j - 2, k) / 3.0;
+                    CONS(b, i, j, k) = EXTRAP(CONS(b, i, j - 1, k), CONS(b, i, j - 2, k));
         for (k = 1; k <= NPRIM; k++)
             for (i = BF + 1; i <= ni + BF; i++)
                 for (j = nj + BF + 1; j <= nj + 2 * BF; j++)
-                    PRIM(b, i, j, k) = PRIM(b, i, j - 1, k) * 4.0 / 3.0 - PRIM(b, i, j - 2, k) / 3.0;
+                    PRIM(b, i, j, k) = EXTRAP(PRIM(b, i, j - 1, k), PRIM(b, i, j - 2, k));
         break;
     case ORA_BC_REFLECT:
         for (j = nj + BF + 1; j <= nj + 2 * BF; j++)
@@ -549,7 +567,8 @@ static void boundary(ora_block* b)
         break;
     case ORA_BC_DMR_TOP: {
         /* boundary_doublemach_general.f90:65-96: shock position at t = current_time */
-        const double xs = 1.0 / 6.0 + (1.0 + 20.0 * b->current_time) / sqrt(3.0);
+        const double tb = b->current_time + (g_mut == MUT_DMR_TOP_T_NEXT ? b->dt : 0.0);
+        const double xs = 1.0 / 6.0 + (1.0 + 20.0 * tb) / sqrt(3.0);
         for (j = nj + BF + 1; j <= nj + 2 * BF; j++)
             for (i = BF + 1; i <= ni + BF; i++) {
                 double x = ((double)(i - BF) - 0.5) * dx; /* x(index_i-bfsize_i,nj) */
@@ -659,7 +678,7 @@ static void split_fluxes(ora_block* b, int kind /* 0 LLF, 1 GLF, 2 SW */, int wi
             if (kind == 2) {
                 sw_split(gamma, rho, u, v, al, 2, fp, fn);
             } else {
-                double a = kind == 0 ? fabs(u) + al : b->max_eig_i;
+                double a = (kind == 0 && g_mut != MUT_LLF_GLOBAL) ? fabs(u) + al : b->max_eig_i;
                 if (kind == 0) OPS(1);
                 lf_split(u, p, a, c, 2, fp, fn);
             }
@@ -765,7 +784,7 @@ static inline void char_face(const double* l, const double* r, double f[5][4], d
         double lf[5];
         for (int m = 0; m < 5; m++)
             lf[m] = M4(l, 1, k) * f[m][0] + M4(l, 2, k) * f[m][1] + M4(l, 3, k) * f[m][2] + M4(l, 4, k) * f[m][3];
-        cv[k - 1] = weno5z(lf);
+        cv[k - 1] = (g_mut == MUT_CH_HONOURS_RECON && s_cur_recon != 255) ? weno5js(lf) : weno5z(lf);
     }
     /* projection: chars 1,2,4 have 4 non-trivial terms (7 ops), char 3 two (2 ops) per point */
     OPS(5 * (3 * 7 + 2));
@@ -777,6 +796,7 @@ static inline void char_face(const double* l, const double* r, double f[5][4], d
 
 static void recon_characteristic(ora_block* b, int adaptive)
 {
+    s_cur_recon = b->p.recon;
     const int ni = b->p.ni, nj = b->p.nj;
     const double gamma = b->p.gamma, delta = b->p.delta;
     int j, i;
@@ -957,7 +977,8 @@ static void time_rk3(ora_block* b, int irk)
         if (!b->p.if_const_dt) { /* :20-24 */
             double dt_xi = b->p.dx / b->max_eig_i;
             double dt_eta = b->p.dy / b->max_eig_j;
-            b->dt = b->p.cfl * dt_xi * dt_eta / (dt_xi + dt_eta);
+            b->dt = g_mut == MUT_DT_ASSOC ? b->p.cfl * (dt_xi * dt_eta) / (dt_xi + dt_eta)
+                                          : b->p.cfl * dt_xi * dt_eta / (dt_xi + dt_eta);
         } else {
             b->dt = b->p.dt_const;
         }
@@ -979,7 +1000,8 @@ static void time_rk3(ora_block* b, int irk)
                     q = 3.0 / 4.0 * q0 + 1.0 / 4.0 * q1 - 1.0 / 4.0 * (dt / dxi * dh_dxi + dt / deta * dh_deta);
                 } else {
                     double q0 = CTMP(b, i, j, k), q1 = CONS(b, i, j, k);
-                    q = 1.0 / 3.0 * q0 + 2.0 / 3.0 * q1 - 2.0 / 3.0 * (dt / dxi * dh_dxi + dt / deta * dh_deta);
+                    q = 1.0 / 3.0 * q0 + (g_mut == MUT_RK3_WEIGHT ? 1.0 - 1.0 / 3.0 : 2.0 / 3.0) * q1 -
+                        2.0 / 3.0 * (dt / dxi * dh_dxi + dt / deta * dh_deta);
                 }
                 CONS(b, i, j, k) = q;
             }
@@ -987,7 +1009,7 @@ static void time_rk3(ora_block* b, int irk)
     /* 2 differences + stage 1: 4 (dt/dx counted as invariant) ; stages 2/3: 7 */
     OPS((long long)ni * nj * NEQ * (irk == 1 ? 6 : 9));
     if (b->p.if_bodyforce) { /* :71-84, :152-165, :234-247 */
-        double coef = irk == 1 ? dt : (irk == 2 ? 1.0 / 4.0 * dt : SP(2.0 / 3.0) * dt);
+        double coef = irk == 1 ? dt : (irk == 2 ? 1.0 / 4.0 * dt : (g_mut == MUT_BODYFORCE_23 ? 2.0 / 3.0 : SP(2.0 / 3.0)) * dt);
         for (int k = 1; k <= NEQ; k++)
             for (j = BF + 1; j <= nj + BF; j++)
                 for (int i = BF + 1; i <= ni + BF; i++) CONS(b, i, j, k) = CONS(b, i, j, k) + coef * BFRC(b, i, j, k);

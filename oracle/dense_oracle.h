@@ -61,6 +61,8 @@ void ora_set_bodyforce(ora_block* b, const double* bf);
 /* One time step (general_main.f90:26-37): stages x {get_primary, boundary,
  * flux splitting + reconstruction, RK stage}, then step += 1, t += dt. */
 void ora_step(ora_block* b);
+/* Test-only: plant mistake m in the restatement (0 = none); see g_mut in dense_oracle.c. */
+void ora_set_mutation(int m);
 /* Run while (t < total_time && step < total_step) (general_main.f90:23-24). */
 int ora_run(ora_block* b, double total_time, int total_step);
 

@@ -76,8 +76,24 @@ def load(ops: bool = False) -> C.CDLL:
     lib.ora_lr_vec_xi.argtypes = [C.c_double] * 5 + [dp, dp]
     lib.ora_lr_vec_eta.argtypes = [C.c_double] * 5 + [dp, dp]
     lib.ora_op_count.restype = C.c_longlong
+    lib.ora_set_mutation.argtypes = [C.c_int]
     _libs[path] = lib
     return lib
+
+
+class planted_mistake:
+    """Context manager for the mutation tests: the restatement with one planted
+    mistake (g_mut in dense_oracle.c), restored on exit."""
+
+    def __init__(self, m: int):
+        self.m = m
+
+    def __enter__(self):
+        load().ora_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        load().ora_set_mutation(0)
 
 
 def _dptr(a: np.ndarray):

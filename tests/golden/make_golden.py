@@ -44,6 +44,10 @@ CASES += [
     ("sv_101_255", 9, 64, 32, 101, 255, 0.5, 10, {}),
     ("sv_104", 9, 64, 32, 104, 255, 0.5, 10, {}),
     ("sv_105", 9, 64, 32, 105, 255, 0.5, 10, {}),
+    # the characteristic schemes ignore the reconstruction switch (always WENO-Z,
+    # scheme_weno5th_CH.f90:73, scheme_weno5th_SC.f90:24-28)
+    ("dmr_104_recon251", 5, 64, 16, 104, 251, 0.6, 10, {}),
+    ("sv_105_recon253", 9, 64, 32, 105, 253, 0.5, 10, {}),
     ("sedov_101_255", 2, 48, 48, 101, 255, 0.5, 10, {}),
     ("sedov_104", 2, 48, 48, 104, 255, 0.5, 10, {}),
     ("sedov_103_253", 2, 40, 40, 103, 253, 0.5, 10, {}),
