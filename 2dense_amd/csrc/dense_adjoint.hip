@@ -9,11 +9,11 @@
 // pass is the exact transpose of the discrete map:
 //   g2 = 2/3 g3 + 2/3 dt J(U2)^T g3,  g1 = 1/4 g2 + 1/4 dt J(U1)^T g2,
 //   g0 = g1 + dt J(U0)^T g1 + 1/3 g3 + 3/4 g2.
-// J^T g is applied by two kernels: adj_face_kernel back-propagates
-// lambda_h(face) = c dt (g(f) - g(f-1))/dx through the WENO-Z stencils of both
-// flux signs of every component and writes the 40 stencil adjoints of the
-// face; adj_cell_kernel gathers them per cell in a fixed order (deterministic)
-// and applies the transpose of the Lax-Friedrichs flux Jacobian.
+// J^T g is applied by one tiled kernel per stage (adj_tile_kernel): it
+// back-propagates lambda_h(face) = c dt (g(f) - g(f-1))/dx through the WENO-Z
+// stencils of both flux signs of every component, gathers the stencil
+// adjoints per cell in a fixed order (deterministic) and applies the transpose
+// of the Lax-Friedrichs flux Jacobian, all in LDS.
 // Stage states are recomputed segment by segment from checkpoints of U^n taken
 // every `k` steps, with the forward stage kernel (bit-identical to the forward pass).
 #include <cmath>
@@ -137,8 +137,6 @@ struct AdjArgs {
     const double* acc_in;
     double* g_out;
     double* acc_out;
-    double* lamx;      // [40][nj][ni]: x faces
-    double* lamy;      // [40][nj][ni]: y faces
     int ni, nj, ld;
     long long plane;
     double gamma, cdt_dx, cdt_dy;  // c*dt/dx, c*dt/dy of the stage
@@ -146,114 +144,182 @@ struct AdjArgs {
     int use_acc_in, add_acc, write_acc;
 };
 
-__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
-
-// One thread per (face, direction, flux sign); blockIdx.y = 2*dir + sign.
-// lambda_h(f) = c dt (g(f) - g(f-1)) / dx is back-propagated through the
-// WENO-Z stencil of that sign for each component; the 5 stencil adjoints of
-// component k go to plane (sign*5 + m)*4 + k of the direction's face array.
-// Stencils: + side lf(m) = F+(f-3+m), - side lf(m) = F-(f+2-m).
-__global__ void __launch_bounds__(256) adj_face_kernel(AdjArgs a)
+__device__ __forceinline__ int wrapi(int i, int n)
 {
-    const int ni = a.ni, nj = a.nj;
-    const long long nf = (long long)ni * nj;
-    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= nf) return;
-    const bool ydir = blockIdx.y >= 2;
-    const int sign = blockIdx.y & 1;  // 0: F+, 1: F-
-    const int q = (int)idx;
-    const int r = q / ni, cidx = q - r * ni;  // row r, column cidx; face between cells f-1 and f
-    double lam_h[4];
-    const double scale = ydir ? a.cdt_dy : a.cdt_dx;
-    {
-        const long long o1 = (long long)r * a.ld + cidx;
-        const long long o0 = ydir ? (long long)wrapi(r - 1, nj) * a.ld + cidx : (long long)r * a.ld + wrapi(cidx - 1, ni);
-#pragma unroll
-        for (int k = 0; k < 4; k++) lam_h[k] = scale * (a.g[k * a.plane + o1] - a.g[k * a.plane + o0]);
-    }
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+
+// One workgroup per tile of ATX x ATY cells applies c dt J(U)^T g to it, in
+// LDS.  Strips of the tile with a 4-cell periodic halo (x strip: rows of the
+// tile, columns x0-4 .. x0+ATX+3; y strip: columns of the tile, rows
+// y0-4 .. y0+ATY+3) hold U, u, v, p, c and g of each cell.  Four rounds, one
+// per (direction, flux sign) in the order x+, x-, y+, y-:
+//  * faces: one thread per face whose stencil reaches a tile cell.
+//    lambda_h(f) = c dt (g(f) - g(f-1)) / dx is back-propagated through the
+//    WENO-Z stencil of that sign for each component (lf(m) = F+(f-3+m) or
+//    F-(f+2-m)); the 5 stencil adjoints per component go to LDS;
+//  * cells: each tile cell sums the 5 contributions to its split flux from
+//    the faces whose stencils hold it (a fixed order: deterministic) and
+//    applies the transpose of the Lax-Friedrichs split-flux Jacobian.
+// The four parts are summed in round order and the RK combination applied.
+constexpr int ATX = 16, ATY = 8, ANT = 256;
+constexpr int AXW = ATX + 8, AYH = ATY + 8;          // strip extents
+constexpr int AXF = (ATX + 4) * ATY;                 // x faces per round (row-major, ATX+4 per row)
+constexpr int AYF = ATX * (ATY + 4);                 // y faces per round (ATX per face row)
+constexpr int AMAXF = AXF > AYF ? AXF : AYF;
+constexpr int ASX = AXW * ATY, ASY = ATX * AYH;      // strip cells
+static_assert(ATX * ATY <= ANT, "one cell per thread");
+
+struct AdjLds {
+    // strip records, slot-major: U0..U3, u, v, p, c, g0..g3
+    double sx[12][ASX];
+    double sy[12][ASY];
+    double fb[20][AMAXF];  // stencil adjoints of the round's faces: [m*4 + k][face]
+};
+
+template <int DIR>
+__device__ __forceinline__ void adj_face_round(const AdjArgs& a, AdjLds& L, int sign, int tid)
+{
+    constexpr int NF = DIR == 0 ? AXF : AYF;
+    constexpr int NRM = DIR == 0 ? 1 : 2;
+    auto S = [&](int slot, int c) -> double {
+        if constexpr (DIR == 0) return L.sx[slot][c];
+        else return L.sy[slot][c];
+    };
+    const double scale = DIR == 0 ? a.cdt_dx : a.cdt_dy;
     const double sg = sign ? -1.0 : 1.0;
-    double U[5][4], w[5], pp[5], unp[5];
-#pragma unroll
-    for (int m = 0; m < 5; m++) {
-        const int off = sign ? 2 - m : m - 3;  // stencil cell of lf(m), relative to f
-        const long long o = ydir ? (long long)wrapi(r + off, nj) * a.ld + cidx
-                                 : (long long)r * a.ld + wrapi(cidx + off, ni);
-#pragma unroll
-        for (int k = 0; k < 4; k++) U[m][k] = a.U[k * a.plane + o];
-        const CellPrim cp = cell_prim(a.gamma, U[m]);
-        const double un = ydir ? cp.v : cp.u;
-        w[m] = un + sg * (fabs(un) + cp.c);
-        pp[m] = cp.p;
-        unp[m] = un * cp.p;
-    }
-    const int NRM = ydir ? 2 : 1;
-    double* out = ydir ? a.lamy : a.lamx;
-    const long long fo = (long long)r * ni + cidx;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
+    // work items (face, component): the 4 components of a face are independent
+    for (int item = tid; item < 4 * NF; item += ANT) {
+        const int k = item / NF, face = item - k * NF;
+        // strip coordinate along DIR of cell f (face f lies between cells f-1 and f)
+        // and the other coordinate
+        int pos, lane;
+        if (DIR == 0) {
+            const int row = face / (ATX + 4), fi = face - row * (ATX + 4);
+            pos = fi + (sign ? 2 : 3);  // f = x0 + fi - (sign ? 2 : 1); strip column of f = f - x0 + 4
+            lane = row;
+        } else {
+            const int fr = face / ATX, col = face - fr * ATX;
+            pos = fr + (sign ? 2 : 3);
+            lane = col;
+        }
+        auto at = [&](int p) { return DIR == 0 ? lane * AXW + p : p * ATX + lane; };
+        const double lam_h = scale * (S(8 + k, at(pos)) - S(8 + k, at(pos - 1)));
         double lf[5], ll[5] = {0, 0, 0, 0, 0};
 #pragma unroll
         for (int m = 0; m < 5; m++) {
-            const double ext = k == NRM ? pp[m] : (k == 3 ? unp[m] : 0.0);
-            lf[m] = 0.5 * (w[m] * U[m][k] + ext);
+            const int c = at(pos + (sign ? 2 - m : m - 3));
+            const double un = S(4 + DIR, c);
+            const double w = un + sg * (fabs(un) + S(7, c));
+            const double ext = k == NRM ? S(6, c) : (k == 3 ? un * S(6, c) : 0.0);
+            lf[m] = 0.5 * (w * S(k, c) + ext);
         }
-        weno5z_vjp(lf, lam_h[k], ll);
+        weno5z_vjp(lf, lam_h, ll);
 #pragma unroll
-        for (int m = 0; m < 5; m++) out[(long long)((sign * 5 + m) * 4 + k) * nf + fo] = ll[m];
+        for (int m = 0; m < 5; m++) L.fb[m * 4 + k][face] = ll[m];
     }
 }
 
-// Four threads per cell, one per (direction, sign): gather the 5 stencil
-// adjoints of the cell's split flux from the faces whose stencils hold it (a
-// fixed order: deterministic), apply the transpose of the split-flux Jacobian,
-// then sum the four parts in a fixed order and apply the RK combination.
-// Block = 64 cells x 4 parts; part = threadIdx.x / 64.
-__global__ void __launch_bounds__(256) adj_cell_kernel(AdjArgs a)
+// The split-flux part of tile cell (lx, ly) for one round.
+template <int DIR>
+__device__ __forceinline__ void adj_cell_round(const AdjArgs& a, const AdjLds& L, int sign, int lx, int ly,
+                                               const double* U, const CellPrim& cp, double* part)
 {
-    __shared__ double part_l[4][4][64];
-    const int ni = a.ni, nj = a.nj;
-    const long long nf = (long long)ni * nj;
-    const int part = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const long long idx = (long long)blockIdx.x * 64 + lane;
-    const bool live = idx < nf;
-    const int j = live ? (int)(idx / ni) : 0, i = live ? (int)(idx - (long long)j * ni) : 0;
-    const long long o = (long long)j * a.ld + i;
-    double lU[4] = {0, 0, 0, 0};
-    if (live) {
-        double U[4];
+    double lF[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) U[k] = a.U[k * a.plane + o];
-        const CellPrim cp = cell_prim(a.gamma, U);
-        const bool ydir = part >= 2;
-        const int sign = part & 1;
-        const double* lam = ydir ? a.lamy : a.lamx;
-        double lF[4];
+    for (int k = 0; k < 4; k++) {
+        double sum = 0.0;
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+            // + sign: the cell is lf(m) of face i+3-m; - sign: of face i-2+m
+            int fidx;
+            if (DIR == 0) fidx = ly * (ATX + 4) + (sign ? lx + m : lx + 4 - m);
+            else fidx = (sign ? ly + m : ly + 4 - m) * ATX + lx;
+            sum += L.fb[m * 4 + k][fidx];
+        }
+        lF[k] = sum;
+    }
+    lf_flux_vjp<DIR>(a.gamma, U, cp, sign ? -1.0 : 1.0, lF, part);
+}
+
+__global__ void __launch_bounds__(ANT) adj_tile_kernel(AdjArgs a)
+{
+    __shared__ AdjLds L;
+    const int tid = threadIdx.x;
+    const int ni = a.ni, nj = a.nj;
+    const int x0 = blockIdx.x * ATX, y0 = blockIdx.y * ATY;
+    // strips: U, primitives (the same cell_prim for every use of a cell), g
+    for (int e = tid; e < ASX + ASY; e += ANT) {
+        int gx, gy, q;
+        double(*S)[ASX] = nullptr;
+        double(*T)[ASY] = nullptr;
+        if (e < ASX) {
+            const int row = e / AXW, col = e - row * AXW;
+            gx = wrapi(x0 + col - 4, ni);
+            gy = wrapi(y0 + row, nj);
+            S = L.sx;
+            q = e;
+        } else {
+            const int r = e - ASX, row = r / ATX, col = r - row * ATX;
+            gx = wrapi(x0 + col, ni);
+            gy = wrapi(y0 + row - 4, nj);
+            T = L.sy;
+            q = r;
+        }
+        const long long o = (long long)gy * a.ld + gx;
+        double U[4], G[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            double sum = 0.0;
-#pragma unroll
-            for (int m = 0; m < 5; m++) {
-                // + sign: this cell is lf(m) of face i+3-m; - sign: of face i-2+m
-                const int f = sign ? m - 2 : 3 - m;
-                const long long fo = ydir ? (long long)wrapi(j + f, nj) * ni + i : (long long)j * ni + wrapi(i + f, ni);
-                sum += lam[(long long)((sign * 5 + m) * 4 + k) * nf + fo];
-            }
-            lF[k] = sum;
+            U[k] = a.U[k * a.plane + o];
+            G[k] = a.g[k * a.plane + o];
         }
-        if (ydir) lf_flux_vjp<1>(a.gamma, U, cp, sign ? -1.0 : 1.0, lF, lU);
-        else lf_flux_vjp<0>(a.gamma, U, cp, sign ? -1.0 : 1.0, lF, lU);
-    }
+        const CellPrim cp = cell_prim(a.gamma, U);
+        const double rec[12] = {U[0], U[1], U[2], U[3], cp.u, cp.v, cp.p, cp.c, G[0], G[1], G[2], G[3]};
 #pragma unroll
-    for (int k = 0; k < 4; k++) part_l[part][k][lane] = lU[k];
+        for (int s = 0; s < 12; s++) {
+            if (S) S[s][q] = rec[s];
+            else T[s][q] = rec[s];
+        }
+    }
     __syncthreads();
-    // thread (k = part, lane): component k of cell lane
+
+    // this thread's tile cell (threads beyond the tile or the grid have none)
+    const int lx = tid % ATX, ly = tid / ATX;
+    const bool live = tid < ATX * ATY && x0 + lx < ni && y0 + ly < nj;
+    double U[4] = {1.0, 0.0, 0.0, 1.0};
+    CellPrim cp{};
+    if (live) {
+        const int c = ly * AXW + lx + 4;
+#pragma unroll
+        for (int k = 0; k < 4; k++) U[k] = L.sx[k][c];
+        cp = cell_prim(a.gamma, U);
+    }
+    double l[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int round = 0; round < 4; round++) {
+        const int sign = round & 1;
+        if (round < 2) adj_face_round<0>(a, L, sign, tid);
+        else adj_face_round<1>(a, L, sign, tid);
+        __syncthreads();
+        if (live) {
+            double part[4] = {0, 0, 0, 0};
+            if (round < 2) adj_cell_round<0>(a, L, sign, lx, ly, U, cp, part);
+            else adj_cell_round<1>(a, L, sign, lx, ly, U, cp, part);
+#pragma unroll
+            for (int k = 0; k < 4; k++) l[k] = round == 0 ? part[k] : l[k] + part[k];
+        }
+        __syncthreads();  // the next round rewrites the face buffer
+    }
     if (!live) return;
-    const int k = part;
-    const double l = ((part_l[0][k][lane] + part_l[1][k][lane]) + part_l[2][k][lane]) + part_l[3][k][lane];
-    const double g = a.g[k * a.plane + o];
-    const double acc = a.use_acc_in ? a.acc_in[k * a.plane + o] : 0.0;
-    a.g_out[k * a.plane + o] = a.add_acc ? a.bcoef * g + l + acc : a.bcoef * g + l;
-    if (a.write_acc) a.acc_out[k * a.plane + o] = acc + a.acoef * g;
+    const long long o = (long long)(y0 + ly) * a.ld + (x0 + lx);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double g = L.sx[8 + k][ly * AXW + lx + 4];
+        const double acc = a.use_acc_in ? a.acc_in[k * a.plane + o] : 0.0;
+        a.g_out[k * a.plane + o] = a.add_acc ? a.bcoef * g + l[k] + acc : a.bcoef * g + l[k];
+        if (a.write_acc) a.acc_out[k * a.plane + o] = acc + a.acoef * g;
+    }
 }
 
 // fresh wave-speed ring and queues before a forward or recomputed segment
@@ -287,7 +353,6 @@ struct dense_adjoint {
     double* ckpt = nullptr;  // nseg states
     double* seg[2] = {nullptr, nullptr};  // stage states (3k+1 each) of two segments: one reversed, one recomputed
     double* gb = nullptr;    // g, g_next, acc (3 states)
-    double* lam = nullptr;   // 80 face planes
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t ready[2] = {nullptr, nullptr}, freed[2] = {nullptr, nullptr}, join = nullptr;
     double fwd_ms = 0.0, bwd_ms = 0.0;
@@ -427,7 +492,6 @@ extern "C" int dense_adj_create(const dense_cfg* cfg, int nsteps, int checkpoint
     a->tile = resolve_tile(c.tile, c.splitting, (long long)c.ni * c.nj);
     a->ntiles = tile_count(a->tile, c.ni, c.nj, &a->tiles_x);
     const size_t st_bytes = sizeof(double) * 4 * a->plane;
-    const size_t nf = (size_t)c.ni * c.nj;
     if (hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&a->rstream, hipStreamNonBlocking) != hipSuccess)
         return bail(afail(nullptr, DENSE_ERR_HIP, "stream creation failed"));
@@ -444,7 +508,7 @@ extern "C" int dense_adj_create(const dense_cfg* cfg, int nsteps, int checkpoint
         hipMalloc(&a->d_dts, sizeof(double) * nsteps) != hipSuccess || hipMalloc(&a->U, 3 * st_bytes) != hipSuccess ||
         hipMalloc(&a->ckpt, (size_t)a->nseg * st_bytes) != hipSuccess ||
         hipMalloc(&a->seg[0], seg_bytes) != hipSuccess || hipMalloc(&a->seg[1], seg_bytes) != hipSuccess ||
-        hipMalloc(&a->gb, 3 * st_bytes) != hipSuccess || hipMalloc(&a->lam, sizeof(double) * 80 * nf) != hipSuccess)
+        hipMalloc(&a->gb, 3 * st_bytes) != hipSuccess)
         return bail(afail(nullptr, DENSE_ERR_NOMEM, "adjoint allocation failed"));
     if (hipMemset(a->U, 0, 3 * st_bytes) != hipSuccess || hipMemset(a->seg[0], 0, seg_bytes) != hipSuccess ||
         hipMemset(a->seg[1], 0, seg_bytes) != hipSuccess || hipMemset(a->gb, 0, 3 * st_bytes) != hipSuccess)
@@ -459,7 +523,7 @@ extern "C" int dense_adj_destroy(dense_adj_handle a)
     if (!a) return DENSE_OK;
     if (a->stream) (void)hipStreamSynchronize(a->stream);
     if (a->rstream) (void)hipStreamSynchronize(a->rstream);
-    double* bufs[] = {a->dt_hist, a->d_dts, a->U, a->ckpt, a->seg[0], a->seg[1], a->gb, a->lam};
+    double* bufs[] = {a->dt_hist, a->d_dts, a->U, a->ckpt, a->seg[0], a->seg[1], a->gb};
     for (double* p : bufs)
         if (p) (void)hipFree(p);
     if (a->st) (void)hipFree(a->st);
@@ -549,8 +613,6 @@ static int adj_vjp(dense_adjoint* a, const double* Us, const double* g, double* 
     x.acc_in = acc_in;
     x.g_out = gout;
     x.acc_out = acc_out;
-    x.lamx = a->lam;
-    x.lamy = a->lam + (size_t)40 * c.ni * c.nj;
     x.ni = c.ni;
     x.nj = c.nj;
     x.ld = a->ld;
@@ -563,10 +625,8 @@ static int adj_vjp(dense_adjoint* a, const double* Us, const double* g, double* 
     x.use_acc_in = acc_in != nullptr;
     x.add_acc = add_acc;
     x.write_acc = acc_out != nullptr;
-    const long long nf = (long long)c.ni * c.nj;
-    hipLaunchKernelGGL(adj_face_kernel, dim3((unsigned)((nf + 255) / 256), 4), dim3(256), 0, a->stream, x);
-    ACHK(a, hipGetLastError());
-    hipLaunchKernelGGL(adj_cell_kernel, dim3((unsigned)((nf + 63) / 64)), dim3(256), 0, a->stream, x);
+    const dim3 grid((unsigned)((c.ni + ATX - 1) / ATX), (unsigned)((c.nj + ATY - 1) / ATY));
+    hipLaunchKernelGGL(adj_tile_kernel, grid, dim3(ANT), 0, a->stream, x);
     ACHK(a, hipGetLastError());
     return DENSE_OK;
 }

@@ -142,7 +142,7 @@ def test_adjoint_fourier_gradient_vs_central_differences_256():
     g = th.grad.cpu().numpy()
     dts = R.dts.copy()
     fwd_ms, bwd_ms = R.last_ms()
-    print(f"J = {float(J):.6e}; forward {fwd_ms:.2f} ms, backward {bwd_ms:.2f} ms", flush=True)
+    print(f"J = {J.detach().item():.6e}; forward {fwd_ms:.2f} ms, backward {bwd_ms:.2f} ms", flush=True)
 
     def Jat(t):
         uN, _ = R.forward(ic(t), dt_schedule=dts)
