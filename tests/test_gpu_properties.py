@@ -9,7 +9,9 @@ for, for every splitting (and each reconstruction of the component-wise ones):
   of rho, rho u, rho v, E change only by rounding;
 * free stream: a uniform state stays uniform (every cell bitwise equal to
   every other) and equal to the initial state to the rounding of the RK
-  weights.
+  weights;
+* re-read parameters: a solver switched mid-run by dense_update_params
+  continues exactly like a fresh solver started from the same state.
 """
 import numpy as np
 import pytest
@@ -99,3 +101,32 @@ def test_free_stream_stays_uniform(sp, rc):
         assert np.all(got[k] == got[k, 0, 0]), f"component {k} not uniform"
     assert np.allclose(got[:, 0, 0], q, rtol=8 * np.finfo(float).eps, atol=0)
     assert np.all(dts == dts[0])
+
+
+def test_update_params_switch_and_cfl_continue_like_fresh_solvers():
+    """dense_update_params mid-run (the reference re-reads input.in at every
+    save, general_main.f90:53): switching 104 -> 101 (which also moves the
+    automatic tile shape from 16x15 to 32x15 on this grid) and then changing
+    the CFL number continues exactly as fresh solvers started from the
+    intermediate states do."""
+    D = dense()
+    ni, nj, h = 320, 320, 1.0 / 320
+    cons = _rough_periodic_state(ni, nj, h, seed=3)
+    a = D.Solver(_periodic_cfg(ni, nj, h, 104, 255))
+    try:
+        a.set_state(cons)
+        a.step(4)
+        mid1 = a.interior()
+        a.update_params(splitting=101, recon=255)
+        a.step(4)
+        mid2 = a.interior()
+        a.update_params(cfl=0.3)
+        dts_a = [a.step() for _ in range(4)]
+        end = a.interior()
+    finally:
+        a.close()
+    ref2, _ = _run(_periodic_cfg(ni, nj, h, 101, 255), mid1, 4)
+    assert np.array_equal(mid2, ref2)
+    ref3, dts_ref = _run(_periodic_cfg(ni, nj, h, 101, 255, cfl=0.3), mid2, 4)
+    assert np.array_equal(end, ref3)
+    assert np.array_equal(np.array(dts_a), dts_ref)
