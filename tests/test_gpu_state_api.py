@@ -1,5 +1,6 @@
 """State read-back through the C ABI: dense_max_eig, dense_get_state with the
-primitive array, and dense_last_run_gpu_ms.
+primitive array, dense_last_run_gpu_ms, and the report of a state that has
+blown up (DENSE_ERR_NONFINITE).
 
 The expected values are the reference's formulas evaluated with numpy in the
 reference's order (get_primary.f90:46-73): u = m/rho, p = (gamma-1)(E -
@@ -89,3 +90,37 @@ def test_last_run_gpu_ms_counts_stage_launches():
         ms, launches = s.last_run_gpu_ms()
         assert taken == 32 and launches == 3 * 32
         assert 0.0 < ms < 1000.0
+
+
+def test_non_finite_state_is_reported_and_the_handle_recovers():
+    """A state that blows up returns DENSE_ERR_NONFINITE instead of running on
+    (the reference's loop condition silently ends on NaN).  Forced from the
+    host: one NaN in the uploaded state; every kernel still runs to its end."""
+    D = dense()
+    with _vortex_solver(n=64) as s:
+        good = s.interior()
+        bad = good.copy()
+        bad[0, 20, 30] = float("nan")
+        s.set_state(bad)
+        with pytest.raises(D.DenseError) as e:
+            s.step()
+        assert e.value.code == -4
+        s.set_state(bad)
+        s.current_time, s.current_step = 0.0, 0
+        with pytest.raises(D.DenseError) as e:
+            s.run(32)
+        assert e.value.code == -4 and s.current_step <= 1
+        s.set_state(good)
+        s.step(2)
+        assert np.all(np.isfinite(s.interior()))
+    B = __import__("importlib").import_module("2dense_amd.batch")
+    cfgs, states, _ = B.make_ensemble(n=3, seed=4, nmin=16, nmax=32)
+    with B.Batch(cfgs) as b:
+        for i, c in enumerate(states):
+            if i == 1:
+                c = c.copy()
+                c[3, 5, 5] = float("inf")
+            b.set_state(i, c)
+        with pytest.raises(D.DenseError) as e:
+            b.run(4)
+        assert e.value.code == -4 and "instance 1" in str(e.value)
