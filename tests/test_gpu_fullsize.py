@@ -244,3 +244,43 @@ def test_dmr_960x240_to_t_end_vs_reference(oracle):
     assert step == d["step"] and t == d["time"]
     assert len(d["dts"]) == step
     assert np.array_equal(got, ref)
+
+
+def _mirror_defect_x(u):
+    sgn = np.array([1.0, -1.0, 1.0, 1.0])[:, None, None]
+    return np.max(np.abs(u - sgn * u[:, :, ::-1])) / np.max(np.abs(u))
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.skipif(os.environ.get("DENSE_LONG") != "1",
+                    reason="full runs to the configured end times (about 90 s of GPU time); set DENSE_LONG=1")
+def test_full_runs_to_configured_end_times():
+    """The double Mach (1920x480, 104) and Rayleigh-Taylor (1024x4096, 105)
+    workloads run as configured, to t = 0.2 and t = 1.95, on the GPU alone (the
+    reference needs hours for them on the host cores): finite, positive density
+    and pressure, the clock stops at the first t >= t_end (no clipping), and the
+    Rayleigh-Taylor flow keeps its mirror symmetry about x = 1/8 to rounding
+    while the instability is still linear (t < 0.6)."""
+    D = dense()
+    bench = __import__("bench")
+    runs = [("DMR 1920x480", D.make_cfg(1920, 480, benchmark=5, splitting=104, recon=255, cfl=0.6),
+             bench.dmr_initial_state(1920, 480), 0.2, 1.4),
+            ("RT 1024x4096", D.make_cfg(1024, 4096, benchmark=3, splitting=105, recon=255, cfl=0.5),
+             bench.rt_initial_state(1024, 4096), 1.95, 5.0 / 3.0)]
+    for name, cfg, ic, t_end, gamma in runs:
+        with D.Solver(cfg) as s:
+            s.set_state(ic)
+            if name.startswith("RT"):
+                s.run(200000, total_time=0.6)
+                d = _mirror_defect_x(s.interior())
+                print(f"{name}: mirror defect {d:.2e} at t = {s.current_time:.3f}", flush=True)
+                assert d < 1e-10
+            s.run(200000, total_time=t_end)
+            u = s.interior()
+            assert t_end <= s.current_time
+            assert s.run(10, total_time=t_end) == 0  # the clock has passed t_end: nothing more to do
+        rho = u[0]
+        p = (gamma - 1.0) * (u[3] - 0.5 * (u[1] ** 2 + u[2] ** 2) / rho)
+        print(f"{name}: {s.current_step} steps to t = {s.current_time!r}; min rho {rho.min():.3e}, "
+              f"min p {p.min():.3e}", flush=True)
+        assert np.all(np.isfinite(u)) and rho.min() > 0 and p.min() > 0
