@@ -661,6 +661,16 @@ static inline void sw_split(double gamma, double rho, double u, double v, double
     (void)s;
 }
 
+/* Test hooks: the splits of one cell (nrm 2: x, 3: y). */
+void ora_lf_split(double un, double p, double a, const double c[4], int nrm, double fp[4], double fn[4])
+{
+    lf_split(un, p, a, c, nrm, fp, fn);
+}
+void ora_sw_split(double gamma, double rho, double u, double v, double alpha, int nrm, double fp[4], double fn[4])
+{
+    sw_split(gamma, rho, u, v, alpha, nrm, fp, fn);
+}
+
 /* Fill flux_xi+/- over i in [1,ni+8], j in [1,nj] and flux_eta+/- over
  * j in [1,nj+8], i in [1,ni] (y stored transposed). */
 static void split_fluxes(ora_block* b, int kind /* 0 LLF, 1 GLF, 2 SW */, int with_indicator)

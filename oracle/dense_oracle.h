@@ -63,6 +63,10 @@ void ora_set_bodyforce(ora_block* b, const double* bf);
 void ora_step(ora_block* b);
 /* Test-only: plant mistake m in the restatement (0 = none); see g_mut in dense_oracle.c. */
 void ora_set_mutation(int m);
+/* Test hooks: the Lax-Friedrichs and Steger-Warming splits of one cell
+ * (nrm 2: x, 3: y); c = conserved state, a = wave speed used by LF. */
+void ora_lf_split(double un, double p, double a, const double c[4], int nrm, double fp[4], double fn[4]);
+void ora_sw_split(double gamma, double rho, double u, double v, double alpha, int nrm, double fp[4], double fn[4]);
 /* Run while (t < total_time && step < total_step) (general_main.f90:23-24). */
 int ora_run(ora_block* b, double total_time, int total_step);
 

@@ -77,6 +77,8 @@ def load(ops: bool = False) -> C.CDLL:
     lib.ora_lr_vec_eta.argtypes = [C.c_double] * 5 + [dp, dp]
     lib.ora_op_count.restype = C.c_longlong
     lib.ora_set_mutation.argtypes = [C.c_int]
+    lib.ora_lf_split.argtypes = [C.c_double, C.c_double, C.c_double, dp, C.c_int, dp, dp]
+    lib.ora_sw_split.argtypes = [C.c_double] * 5 + [C.c_int, dp, dp]
     _libs[path] = lib
     return lib
 

@@ -130,3 +130,18 @@ def test_update_params_switch_and_cfl_continue_like_fresh_solvers():
     ref3, dts_ref = _run(_periodic_cfg(ni, nj, h, 101, 255, cfl=0.3), mid2, 4)
     assert np.array_equal(end, ref3)
     assert np.array_equal(np.array(dts_a), dts_ref)
+
+
+def test_adaweno_threshold_one_equals_characteristic_at_960x240():
+    """theta <= 1, so AdaWENO with delta = 1 never takes the component-wise
+    branch (scheme_weno5th_SC.f90:110) and equals switch 104 bit for bit; at
+    the default delta = 0.5 it differs.  Double Mach 960x240, 20 steps."""
+    D = dense()
+    bench = __import__("bench")
+    ic = bench.dmr_initial_state(960, 240)
+    outs = {}
+    for sp, delta in ((104, 0.5), (105, 1.0), (105, 0.5)):
+        cfg = D.make_cfg(960, 240, benchmark=5, splitting=sp, recon=255, cfl=0.6, delta=delta)
+        outs[(sp, delta)], _ = _run(cfg, ic, 20)
+    assert np.array_equal(outs[(105, 1.0)], outs[(104, 0.5)])
+    assert not np.array_equal(outs[(105, 0.5)], outs[(104, 0.5)])

@@ -141,3 +141,68 @@ def test_conservation_periodic(sp, rc):
     after = o.interior().sum(axis=(1, 2))
     scale = np.abs(o.interior()).sum(axis=(1, 2))
     assert np.all(np.abs(after - before) <= 1e-12 * scale + 1e-14)
+
+
+def _euler_flux(rho, u, v, p, gamma, nrm):
+    E = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
+    un = u if nrm == 2 else v
+    f = np.array([rho * un, rho * un * u, rho * un * v, un * (E + p)])
+    f[nrm - 1] += p
+    return f, np.array([rho, rho * u, rho * v, E])
+
+
+def _split(lib, kind, rho, u, v, p, gamma, nrm, a=None):
+    c = np.sqrt(gamma * p / rho)
+    fp, fn = np.zeros(4), np.zeros(4)
+    dp = lambda x: x.ctypes.data_as(po.C.POINTER(po.C.c_double))  # noqa: E731
+    if kind == "lf":
+        _, U = _euler_flux(rho, u, v, p, gamma, nrm)
+        un = u if nrm == 2 else v
+        lib.ora_lf_split(un, p, abs(un) + c if a is None else a, dp(U), nrm, dp(fp), dp(fn))
+    else:
+        lib.ora_sw_split(gamma, rho, u, v, c, nrm, dp(fp), dp(fn))
+    return fp, fn
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_flux_splits_add_up_to_the_euler_flux(lib, seed):
+    """Lax-Friedrichs (flux_splitting_LLF.f90:37-47, GLF the same with a global
+    speed) and Steger-Warming (flux_splitting_SW.f90:36-161): F+ + F- = F(U)
+    in both directions; for LF also F+ - F- = alpha U, and Steger-Warming
+    sends a supersonic flow almost entirely to F+ (lambda- = (lambda - sqrt(
+    lambda^2 + eps^2))/2 is O(eps^2/lambda) with eps = 0.125)."""
+    rng = np.random.default_rng(seed)
+    gamma = 1.4
+    for _ in range(20):
+        rho, p = rng.uniform(0.2, 5.0, 2)
+        u, v = rng.uniform(-3.0, 3.0, 2)
+        for nrm in (2, 3):
+            F, U = _euler_flux(rho, u, v, p, gamma, nrm)
+            for kind in ("lf", "sw"):
+                fp, fn = _split(lib, kind, rho, u, v, p, gamma, nrm)
+                assert np.allclose(fp + fn, F, rtol=1e-13, atol=1e-13 * np.abs(F).max()), (kind, nrm)
+            a = rng.uniform(0.0, 10.0)
+            fp, fn = _split(lib, "lf", rho, u, v, p, gamma, nrm, a=a)
+            assert np.allclose(fp - fn, a * U, rtol=1e-13, atol=1e-13 * np.abs(a * U).max())
+    rho, p = 1.0, 1.0
+    c = np.sqrt(gamma)
+    F, _ = _euler_flux(rho, 8.0 * c, 0.3, p, gamma, 2)
+    fp, fn = _split(lib, "sw", rho, 8.0 * c, 0.3, p, gamma, 2)
+    assert np.abs(fn).max() < 1e-3 * np.abs(F).max()
+
+
+def test_adaweno_threshold_one_is_the_characteristic_scheme():
+    """theta = 1/(1 + (sum alpha - 1)^2) <= 1 (weno_mod.f90:94), so with
+    delta >= 1 no face takes the component-wise branch (theta > delta,
+    scheme_weno5th_SC.f90:110) and switch 105 is switch 104 bit for bit; with
+    the default delta = 0.5 it is not."""
+    outs = {}
+    for sp, delta in ((104, 0.5), (105, 1.0), (105, 0.5)):
+        p = po.problem_params(5, 64, 16, splitting=sp, recon=255, cfl=0.6, delta=delta)
+        o = po.Oracle(p)
+        o.init_state(5)
+        o.step(8)
+        outs[(sp, delta)] = o.interior()
+        o.close()
+    assert np.array_equal(outs[(105, 1.0)], outs[(104, 0.5)])
+    assert not np.array_equal(outs[(105, 0.5)], outs[(104, 0.5)])
