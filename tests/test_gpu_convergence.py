@@ -1,4 +1,5 @@
-"""Order of accuracy on a smooth flow with an exact solution.
+"""Order of accuracy on a smooth flow with an exact solution, in space and in
+time.
 
 The isentropic vortex in the periodic free stream u = v = 1 is transported
 without change: at time T the exact state is the initial one translated by
@@ -41,3 +42,27 @@ def test_fifth_order_on_the_isentropic_vortex(sp, rc):
     print(f"{sp}/{rc}: L1 {['%.3e' % e for e in errs]}, orders {['%.2f' % o for o in orders]}", flush=True)
     assert all(o > 3.5 for o in orders)
     assert orders[-1] > 4.4
+
+
+@pytest.mark.parametrize("sp,rc", [(101, 255), (104, 255)])
+def test_rk3_third_order_in_time(sp, rc):
+    """SSP-RK3 (time_advance_general.f90:35-217) at a fixed grid: halving a
+    constant step twice, the differences between successive solutions shrink by
+    2^3 (a wrong stage weight would leave first or second order)."""
+    D = dense()
+    B = importlib.import_module("2dense_amd.batch")
+    n, T = 160, 1.0
+    cons, h = B.vortex_state(n, 5.0, 5.0, 5.0)
+    sols = []
+    for steps in (80, 160, 320):  # CFL 0.7, 0.35, 0.18
+        cfg = D.make_cfg(n, n, benchmark=0, splitting=sp, recon=rc, gamma=1.4, dx=h, dy=h,
+                         bc=[D.BC_PERIODIC] * 4, if_const_dt=True, dt=T / steps)
+        with D.Solver(cfg) as s:
+            s.set_state(cons)
+            assert s.run(steps) == steps
+            sols.append(s.interior())
+    d1 = np.sum(np.abs(sols[0] - sols[1]))
+    d2 = np.sum(np.abs(sols[1] - sols[2]))
+    order = math.log2(d1 / d2)
+    print(f"{sp}/{rc}: time order {order:.2f}", flush=True)
+    assert 2.6 < order < 3.4
