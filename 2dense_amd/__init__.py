@@ -93,12 +93,13 @@ def load_library(path: str | None = None) -> C.CDLL:
     lib.dense_max_eig.argtypes = [H, dp]
     lib.dense_last_run_gpu_ms.argtypes = [H, dp, C.POINTER(C.c_int)]
     lib.dense_update_params.argtypes = [H, P]
+    lib.dense_autotune.argtypes = [H, C.c_int, C.POINTER(C.c_int), dp]
     lib.dense_sync.argtypes = [H]
     lib.dense_last_error.argtypes = [H]
     lib.dense_last_error.restype = C.c_char_p
     for name in ("dense_problem_defaults", "dense_create", "dense_destroy", "dense_set_state",
                  "dense_get_state", "dense_step", "dense_run", "dense_max_eig", "dense_last_run_gpu_ms", "dense_update_params",
-                 "dense_sync"):
+                 "dense_autotune", "dense_sync"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -254,6 +255,16 @@ class Solver:
         for k, v in kw.items():
             setattr(self.cfg, k, v)
         _check(self.lib, self.h, self.lib.dense_update_params(self.h, C.byref(self.cfg)))
+
+    def autotune(self, trial_steps: int = 16):
+        """Time each tile shape on trial_steps steps of the current state (which
+        is restored) and keep the fastest; returns (cfg.tile value, trial ms).
+        Runs replay 16-step graphs, so trials of 16 steps time whole replays."""
+        t = C.c_int(0)
+        ms = C.c_double(0.0)
+        _check(self.lib, self.h, self.lib.dense_autotune(self.h, trial_steps, C.byref(t), C.byref(ms)))
+        self.cfg.tile = t.value
+        return t.value, ms.value
 
     def sync(self):
         _check(self.lib, self.h, self.lib.dense_sync(self.h))

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <climits>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -553,6 +554,61 @@ extern "C" int dense_update_params(dense_handle h, const dense_cfg* cfg)
     HIPCHK(h, hipMemcpyAsync(h->st, &z, sizeof z, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, launch_max(a, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    return DENSE_OK;
+}
+
+// Time every tile shape on trial_steps steps of the current state (state and
+// clock restored after each trial) and keep the fastest as an explicit
+// cfg.tile.  Every shape gives the same bits; which is fastest depends on how
+// the grid's tile count fills the resident workgroups, not only on its size.
+extern "C" int dense_autotune(dense_handle h, int trial_steps, int* tile, double* best_ms)
+{
+    if (!h || trial_steps < 1) return fail(h, DENSE_ERR_ARG, "bad arguments");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    int rc = read_state(h);
+    if (rc) return rc;
+    const DevState st0 = *h->st_host;
+    const size_t bytes = sizeof(double) * 4 * h->plane;
+    double* save = nullptr;
+    HIPCHK(h, hipMalloc(&save, bytes));
+    auto restore = [&]() -> int {
+        HIPCHK(h, hipMemcpyAsync(h->U[0], save, bytes, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->st, &st0, sizeof st0, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        return DENSE_OK;
+    };
+    auto use_tile = [&](int t) {
+        h->cfg.tile = t;
+        h->tile = resolve_tile(t, h->cfg.splitting, (long long)h->cfg.ni * h->cfg.nj);
+        h->ntiles = tile_count(h->tile, h->cfg.ni, h->cfg.nj, &h->tiles_x);
+        release_graph(h);
+    };
+    const int keep = h->cfg.tile;
+    double best = 1e300;
+    int bt = 0;
+    if (hipMemcpyAsync(save, h->U[0], bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        rc = fail(h, DENSE_ERR_HIP, "state snapshot failed");
+    for (int t : {4, 3, 1, 2}) {  // 16x15, 32x15, 32x7, 8x7
+        if (rc) break;
+        use_tile(t);
+        for (int pass = 0; pass < 2 && !rc; pass++) {  // the first pass builds the graph
+            double tt = st0.t;
+            int ss = st0.step, taken = 0;
+            if (!(rc = restore())) rc = dense_run(h, trial_steps, INFINITY, INT_MAX, &tt, &ss, nullptr, &taken);
+        }
+        if (!rc && h->last_run_ms < best) {
+            best = h->last_run_ms;
+            bt = t;
+        }
+    }
+    use_tile(rc ? keep : bt);
+    const int rr = restore();
+    (void)hipFree(save);
+    if (rc) return rc;
+    if (rr) return rr;
+    if (tile) *tile = bt;
+    if (best_ms) *best_ms = best;
     return DENSE_OK;
 }
 

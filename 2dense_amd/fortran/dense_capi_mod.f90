@@ -63,6 +63,14 @@ module dense_capi
       type(dense_cfg), intent(in) :: cfg
     end function
 
+    integer(c_int) function dense_autotune(h, trial_steps, tile, best_ms) bind(C, name="dense_autotune")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: trial_steps
+      integer(c_int), intent(out) :: tile
+      real(c_double), intent(out) :: best_ms
+    end function
+
     type(c_ptr) function dense_last_error(h) bind(C, name="dense_last_error")
       import :: c_ptr
       type(c_ptr), value :: h

@@ -104,10 +104,14 @@ def rt_initial_state(ni: int, nj: int, gamma: float = 5.0 / 3.0) -> np.ndarray:
     return np.stack([rho, np.zeros_like(rho), rho * v, p / (gamma - 1.0) + 0.5 * rho * v * v])
 
 
+TILE_NAMES = {1: "32x7", 2: "8x7", 3: "32x15", 4: "16x15"}
+
+
 def workloads_leg(reps: int = 3):
     """GPU throughput of the other configured workloads (parity cases, not the
-    metric): the first run captures the graphs, then `reps` timed runs of the
-    same steps from the same initial state; best wall time, cell-updates/s."""
+    metric): tile shape from dense_autotune, then a first run that captures the
+    graphs and `reps` timed runs of the same steps from the same initial state;
+    best wall time, cell-updates/s."""
     D = importlib.import_module("2dense_amd")
     B = importlib.import_module("2dense_amd.batch")
     vort, h = B.vortex_state(200, 5.0, 5.0, 5.0)
@@ -126,6 +130,8 @@ def workloads_leg(reps: int = 3):
     for name, cfg, cons, steps in cases:
         best = None
         with D.Solver(cfg) as s:
+            s.set_state(cons)
+            tile, _ = s.autotune(16)  # the same bits for every shape
             for r in range(reps + 1):
                 s.set_state(cons)
                 s.current_time, s.current_step = 0.0, 0
@@ -139,7 +145,7 @@ def workloads_leg(reps: int = 3):
                 if r > 0 and (best is None or wall < best[0]):
                     best = (wall, s.last_run_gpu_ms()[0])
         cells = cfg.ni * cfg.nj
-        out.append(dict(workload=name, ni=cfg.ni, nj=cfg.nj, steps=steps,
+        out.append(dict(workload=name, ni=cfg.ni, nj=cfg.nj, steps=steps, tile=TILE_NAMES[tile],
                         cell_updates_per_s=cells * steps / best[0], gpu_cell_updates_per_s=cells * steps / (best[1] * 1e-3),
                         ms_per_step=best[0] / steps * 1e3))
     return out

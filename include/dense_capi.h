@@ -131,6 +131,13 @@ int dense_max_eig(dense_handle h, double out[2]);
  * at every save. */
 int dense_update_params(dense_handle h, const dense_cfg* cfg);
 
+/* Time each tile shape (cfg.tile 4, 3, 1, 2) on trial_steps steps of the
+ * current state, restoring state and clock after every trial, and keep the
+ * fastest as an explicit cfg.tile (dense_update_params keeps it).  *tile and
+ * *best_ms (nullable) receive the choice and its trial time.  No counterpart in
+ * the reference (one CPU code path); all shapes give the same bits. */
+int dense_autotune(dense_handle h, int trial_steps, int* tile, double* best_ms);
+
 /* Wait for queued device work of this handle. */
 int dense_sync(dense_handle h);
 

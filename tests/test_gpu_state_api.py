@@ -124,3 +124,26 @@ def test_non_finite_state_is_reported_and_the_handle_recovers():
         with pytest.raises(D.DenseError) as e:
             b.run(4)
         assert e.value.code == -4 and "instance 1" in str(e.value)
+
+
+@pytest.mark.parametrize("splitting", [104, 101])
+def test_autotune_restores_state_and_clock_and_keeps_the_bits(splitting):
+    """dense_autotune runs trial steps with every tile shape, then restores the
+    state and the clock; the run after it equals a run with the default shape
+    bit for bit, and the chosen shape is kept as an explicit cfg.tile."""
+    with _vortex_solver(splitting=splitting) as a, _vortex_solver(splitting=splitting) as b:
+        a.step(2)
+        b.step(2)
+        before = a.interior()
+        t0, s0 = a.current_time, a.current_step
+        eig0 = a.max_eig()
+        tile, ms = a.autotune(trial_steps=3)
+        assert tile in (1, 2, 3, 4) and ms > 0
+        assert a.cfg.tile == tile
+        assert np.array_equal(a.interior(), before)
+        assert a.max_eig() == eig0
+        a.current_time, a.current_step = t0, s0
+        a.run(7)
+        b.run(7)
+        assert (a.current_time, a.current_step) == (b.current_time, b.current_step)
+        assert np.array_equal(a.interior(), b.interior())
