@@ -326,9 +326,9 @@ __global__ void __launch_bounds__(ANT) adj_tile_kernel(AdjArgs a)
 __global__ void adj_reset_kernel(DevState* st, int hist_cap)
 {
     for (int s = 0; s < 4; s++) st->alpha[s][0] = st->alpha[s][1] = 0ull;
-    for (int x = 0; x < 8; x++) st->qnext[x] = 0u;
+    for (int x = 0; x < 8; x++) st->qnext[x].v = 0u;
     st->seq = 0;
-    st->blocks_done = 0u;
+    st->blocks_done.v = 0u;
     st->error = 0;
     st->active = 0;
     st->hist_cap = hist_cap;

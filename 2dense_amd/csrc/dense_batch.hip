@@ -35,9 +35,9 @@ batch_stage_kernel(BatchArgs b)
     const int xq = blockIdx.x & 7;
     int lo, hi;
     queue_range(xq, b.ntiles, lo, hi);
-    if (tid == 0) s_tile = lo + (int)atomicAdd(&bs->qnext[xq], 1u);
-    __syncthreads();
-    int tile = s_tile;
+    // first tile: the workgroup's rank on its XCD; the queue hands out the rest
+    int tile = lo + (int)(blockIdx.x >> 3);
+    const int rest = lo + (int)((gridDim.x + 7 - xq) >> 3);
     int cur = -1;
     unsigned long long ei = 0, ej = 0;
     while (tile < hi) {
@@ -84,7 +84,7 @@ batch_stage_kernel(BatchArgs b)
         const int lt = tile - b.inst[inst].tile_base;
         if (b.stage == 1 && lt == 0 && tid == 0) b.ist[inst].dt = s_dt;  // every tile computes the same dt
         run_tile<SCHEME, T>(sa, lt, s_ai, s_aj, s_dt, s_t, L, ei, ej,
-                            [&] { s_tile = lo + (int)atomicAdd(&bs->qnext[xq], 1u); });
+                            [&] { s_tile = rest + (int)atomicAdd(&bs->qnext[xq].v, 1u); });
         tile = s_tile;
     }
     if (cur >= 0) block_max_to_slot<T::NT>(ei, ej, b.ist[cur].alpha[(seq + 1) & 3]);
@@ -94,7 +94,7 @@ batch_stage_kernel(BatchArgs b)
     if (tid == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         const unsigned int prev =
-            __hip_atomic_fetch_add(&bs->blocks_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&bs->blocks_done.v, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_last = (prev == gridDim.x - 1);
     }
     __syncthreads();
@@ -115,9 +115,9 @@ batch_stage_kernel(BatchArgs b)
         }
     }
     if (tid == 0) {
-        __hip_atomic_store(&bs->blocks_done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&bs->blocks_done.v, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
-        for (int x = 0; x < 8; x++) __hip_atomic_store(&bs->qnext[x], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int x = 0; x < 8; x++) __hip_atomic_store(&bs->qnext[x].v, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         bs->seq = seq + 1;
         if (b.last_stage) {
             bs->step += 1;
@@ -353,8 +353,8 @@ static int batch_prime(dense_batch* b)
     BatchState bs{};
     BCHK(b, hipMemcpy(&bs, b->d_bst, sizeof bs, hipMemcpyDeviceToHost));
     bs.seq = 0;
-    bs.blocks_done = 0;
-    for (auto& q : bs.qnext) q = 0;
+    bs.blocks_done.v = 0;
+    for (auto& q : bs.qnext) q.v = 0;
     bs.active = 0;
     BCHK(b, hipMemcpy(b->d_bst, &bs, sizeof bs, hipMemcpyHostToDevice));
     BatchArgs a = batch_args(b);

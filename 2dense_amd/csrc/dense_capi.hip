@@ -350,7 +350,7 @@ extern "C" int dense_set_state(dense_handle h, const double* cons)
     DevState z = *h->st_host;
     std::memset(z.alpha, 0, sizeof z.alpha);
     z.seq = 0;
-    z.blocks_done = 0;
+    z.blocks_done.v = 0;
     z.error = 0;
     z.active = 0;
     HIPCHK(h, hipMemcpyAsync(h->st, &z, sizeof z, hipMemcpyHostToDevice, h->stream));
@@ -550,7 +550,7 @@ extern "C" int dense_update_params(dense_handle h, const dense_cfg* cfg)
     DevState z = *h->st_host;
     std::memset(z.alpha, 0, sizeof z.alpha);
     z.seq = 0;
-    z.blocks_done = 0;
+    z.blocks_done.v = 0;
     HIPCHK(h, hipMemcpyAsync(h->st, &z, sizeof z, hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, launch_max(a, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));

@@ -17,28 +17,38 @@ stage_kernel(StageArgs a)
     __shared__ TileLds<T> L;
     __shared__ int s_tile;
     DevState* st = a.st;
-    if (!st->active) return;
-    const int seq = st->seq;
-    const double ai = __longlong_as_double((long long)st->alpha[seq & 3][0]);
-    const double aj = __longlong_as_double((long long)st->alpha[seq & 3][1]);
+    // Every word this launch reads from the device state is loaded at once (the
+    // whole wave-speed ring beside seq): one memory round trip, not three.
+    unsigned long long ring[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) ring[k] = (&st->alpha[0][0])[k];
+    const int active = st->active, seq = st->seq, step = st->step;
+    const double t = st->t, dt_prev = st->dt;
+    if (!active) return;
+    unsigned long long bi = ring[0], bj = ring[1];
+#pragma unroll
+    for (int k = 1; k < 4; k++)
+        if ((seq & 3) == k) bi = ring[2 * k], bj = ring[2 * k + 1];
+    const double ai = __longlong_as_double((long long)bi);
+    const double aj = __longlong_as_double((long long)bj);
     double dt;
     if (a.stage == 1) {
-        dt = a.dt_sched ? a.dt_sched[st->step] : cfl_dt(a, ai, aj);
+        dt = a.dt_sched ? a.dt_sched[step] : cfl_dt(a, ai, aj);
         if (blockIdx.x == 0 && threadIdx.x == 0) st->dt = dt;
     } else {
-        dt = st->dt;
+        dt = dt_prev;  // set by workgroup 0 of this step's first stage
     }
-    const double t = st->t;
     const int xq = blockIdx.x & 7;
     int lo, hi;
     queue_range(xq, a.ntiles, lo, hi);
-    if (threadIdx.x == 0) s_tile = lo + (int)atomicAdd(&st->qnext[xq], 1u);
-    __syncthreads();
-    int tile = s_tile;
+    // The first tile of a workgroup is its rank among the workgroups of its
+    // XCD (no atomic before the first load); the queue hands out the rest.
+    int tile = lo + (int)(blockIdx.x >> 3);
+    const int rest = lo + (int)((gridDim.x + 7 - xq) >> 3);
     unsigned long long ei = 0, ej = 0;
     while (tile < hi) {
         run_tile<SCHEME, T>(a, tile, ai, aj, dt, t, L, ei, ej,
-                            [&] { s_tile = lo + (int)atomicAdd(&st->qnext[xq], 1u); });
+                            [&] { s_tile = rest + (int)atomicAdd(&st->qnext[xq].v, 1u); });
         tile = s_tile;
     }
     block_max_to_slot<T::NT>(ei, ej, st->alpha[(seq + 1) & 3]);

@@ -109,11 +109,11 @@ __device__ void launch_epilogue(DevState* st, int seq, bool end_of_step, double 
     if (threadIdx.x != 0) return;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     const unsigned int prev =
-        __hip_atomic_fetch_add(&st->blocks_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&st->blocks_done.v, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (prev != gridDim.x - 1) return;
-    __hip_atomic_store(&st->blocks_done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&st->blocks_done.v, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
-    for (int x = 0; x < 8; x++) __hip_atomic_store(&st->qnext[x], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int x = 0; x < 8; x++) __hip_atomic_store(&st->qnext[x].v, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int out = (seq + 1) & 3, nxt = (seq + 2) & 3;
     __hip_atomic_store(&st->alpha[nxt][0], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&st->alpha[nxt][1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
