@@ -55,6 +55,21 @@ module driver_state
 
 contains
 
+  ! Whether to pick the stage kernel's tile shape by timing before the run
+  ! (dense_autotune: every shape gives the same bits).  Worth its 128 trial
+  ! steps on long runs only; DENSE_AUTOTUNE=1 or 0 forces it on or off.
+  logical function want_autotune(steps_left)
+    integer, intent(in) :: steps_left
+    character(len=8) :: v
+    integer :: n, st
+    call get_environment_variable('DENSE_AUTOTUNE', v, n, st)
+    if (st == 0 .and. n > 0) then
+      want_autotune = v(1:1) == '1'
+    else
+      want_autotune = steps_left >= 2000
+    end if
+  end function want_autotune
+
   subroutine read_input()
     logical :: there
     inquire(file='input.in', exist=there)
@@ -413,6 +428,8 @@ program dense_gpu
   integer(c_int) :: rc
   integer(8) :: c0, c1, rate
   double precision :: cpu_t, step_dt
+  integer(c_int) :: tile_choice
+  real(c_double) :: trial_ms
 
   call read_input()
   ni = single_ni
@@ -435,6 +452,11 @@ program dense_gpu
   rc = dense_set_state(h, cons)
   call dense_check(rc, h, 'dense_set_state')
   write(*, *) 'Device: MI355X (HIP), FP64'
+  if (want_autotune(total_step - current_step)) then
+    rc = dense_autotune(h, 16, tile_choice, trial_ms)
+    call dense_check(rc, h, 'dense_autotune')
+    write(*, *) 'Tile shape chosen by timing (cfg%tile):', tile_choice
+  end if
   open(20, file='CpuTime.dat', access='append')
 
   cpu_t = 0.d0

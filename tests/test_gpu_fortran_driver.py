@@ -19,14 +19,15 @@ DRIVER = os.path.join(ROOT, "2dense_amd", "bin", "dense_gpu")
 CASES = golden_cases()
 
 
-def _run_driver(tmp_path, c, steps, **extra):
+def _run_driver(tmp_path, c, steps, env=None, **extra):
     import pyoracle as po
     nml = dict(benchmark=c["benchmark"], ni=c["ni"], nj=c["nj"], splitting=c["splitting"], recon=c["recon"],
                cfl=c["cfl"], stages=c.get("stages", 3), if_const_dt=c.get("if_const_dt", False),
                dt=c.get("dt", 0.01), if_bodyforce=c.get("if_bodyforce", False), gamma=c.get("gamma", 1.4))
     nml.update(extra)
     po.write_input_in(str(tmp_path / "input.in"), total_step=steps, save_step=steps, **nml)
-    out = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    out = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=120,
+                         env=None if env is None else {**os.environ, **env})
     assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
     dts = [float(m.group(1).replace("D", "E")) for m in re.finditer(r"dt=\s*(\S+)", out.stdout)]
     return po.read_dat2d(str(tmp_path / f"2d_{steps}.dat2d")), np.array(dts), out.stdout
@@ -62,3 +63,25 @@ def test_driver_restart_roundtrip(tmp_path):
     assert out.returncode == 0, out.stdout[-2000:]
     d = po.read_dat2d(str(tmp_path / "2d_20.dat2d"))
     assert np.array_equal(d["cons"][:, 4:-4, 4:-4], g["cons"])
+
+
+@pytest.mark.parametrize("name", ["dmr_104", "ivc_101_253", "ssl_105_30"])
+def test_driver_autotune_keeps_the_bits(tmp_path, name):
+    """DENSE_AUTOTUNE=1 makes the driver time the tile shapes before its run
+    (dense_autotune restores state and clock); the outputs stay the
+    reference's, bit for bit, and the printed dt sequence is unchanged."""
+    if not os.path.exists(DRIVER):
+        pytest.fail("Fortran driver not built (python 2dense_amd/build.py)")
+    c = CASES[name]
+    steps = c["steps"]
+    print(f"driver autotune case {name}", flush=True)
+    (tmp_path / "on").mkdir()
+    (tmp_path / "off").mkdir()
+    on, dts_on, out_on = _run_driver(tmp_path / "on", c, steps, env={"DENSE_AUTOTUNE": "1"})
+    off, dts_off, out_off = _run_driver(tmp_path / "off", c, steps, env={"DENSE_AUTOTUNE": "0"})
+    assert "Tile shape chosen by timing" in out_on and "Tile shape chosen by timing" not in out_off
+    assert np.array_equal(dts_on, dts_off)
+    g = golden(name)
+    assert np.array_equal(dts_on, g["dts"])
+    for d in (on, off):
+        assert np.array_equal(d["cons"][:, 4:-4, 4:-4], g["cons"])
