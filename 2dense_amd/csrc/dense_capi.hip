@@ -539,7 +539,7 @@ extern "C" int dense_update_params(dense_handle h, const dense_cfg* cfg)
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->cfg = c;
     h->scheme = scheme_of(c.splitting, c.recon);
-    h->tile = resolve_tile(c.tile, c.splitting, (long long)c.ni * c.nj);  // the automatic shape follows the splitting
+    h->tile = resolve_tile(c.tile, c.splitting, (long long)c.ni * c.nj);
     h->ntiles = tile_count(h->tile, c.ni, c.nj, &h->tiles_x);
     release_graph(h);  // kernel arguments changed
     // the wave speeds of the current state depend on gamma

@@ -126,20 +126,16 @@ hipError_t launch_fill_padded(const StageArgs& a, double* cons_pad, double* prim
 int tile_count(int tile, int ni, int nj, int* tiles_x);
 
 // dense_cfg.tile -> index of the tile shape list (dense_tile.h: 0 16x15,
-// 1 32x7, 2 8x7, 3 32x15).  0 chooses by splitting and grid size,
-// from timings on MI355X: the 32x15 tile (fewer halo cells and duplicated
-// faces per cell) is 5-110 % faster for the component-wise Lax-Friedrichs
-// paths (101, 102) and AdaWENO (105) from about 80,000 cells up; the 16x15
-// tile is 8-13 % faster there for Steger-Warming (103) and the characteristic
-// scheme (104), and is the faster shape for every path on smaller grids, where
-// more, smaller workgroups fill the CUs.  1..3 force those shapes and 4 forces
-// the 16x15 shape.
-inline int resolve_tile(int tile, int splitting, long long cells)
+// 1 32x7, 2 8x7, 3 32x15).  0 (automatic) is the 16x15 tile: timed on MI355X
+// for splittings 101, 102, 104 and 105 on 256^2 to 3840x960 grids, it is the
+// fastest shape or within 5 % of it, except for GLF + upwind5 on the smallest
+// grids (32x15 is 4-12 % faster there); the 32x15 tile is up to 31 % slower on
+// large grids (two 256-thread workgroups per CU hide each other's load and
+// barrier phases; one 512-thread workgroup cannot).  dense_autotune times the
+// shapes for a given grid.  1..3 force those shapes and 4 forces 16x15.
+inline int resolve_tile(int tile, int /*splitting*/, long long /*cells*/)
 {
-    if (tile == 4) return 0;
-    if (tile != 0) return tile;
-    const bool lf = splitting == 101 || splitting == 102 || splitting == 105;
-    return (lf && cells >= 80000) ? 3 : 0;
+    return tile == 4 ? 0 : tile;
 }
 
 }  // namespace dense

@@ -66,9 +66,9 @@ typedef struct dense_cfg {
     int benchmark;           /* BENCHMARK_INDEX: 1, 2, 3, 4, 5, 6, 9, or 0 for bc[] below */
     int bc[4];               /* per-side kind (benchmark 0) */
     double bc_state[4][4];   /* per-side Dirichlet (rho,u,v,p); DMR sides: post-shock state */
-    int tile;                /* tile shape: 0 automatic (32x15 for splittings 101, 102, 105 on
-                                grids of >= 80,000 cells, else 16x15), 1 32x7, 2 8x7, 3 32x15,
-                                4 16x15.  Results do not depend on it. */
+    int tile;                /* tile shape: 0 automatic (16x15), 1 32x7, 2 8x7, 3 32x15,
+                                4 16x15; dense_autotune picks one by timing.  Results do
+                                not depend on it. */
 } dense_cfg;
 
 typedef struct dense_solver* dense_handle;

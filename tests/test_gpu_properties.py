@@ -105,10 +105,9 @@ def test_free_stream_stays_uniform(sp, rc):
 
 def test_update_params_switch_and_cfl_continue_like_fresh_solvers():
     """dense_update_params mid-run (the reference re-reads input.in at every
-    save, general_main.f90:53): switching 104 -> 101 (which also moves the
-    automatic tile shape from 16x15 to 32x15 on this grid) and then changing
-    the CFL number continues exactly as fresh solvers started from the
-    intermediate states do."""
+    save, general_main.f90:53): switching 104 -> 101 (another kernel on the
+    same tile shape) and then changing the CFL number continues exactly as
+    fresh solvers started from the intermediate states do."""
     D = dense()
     ni, nj, h = 320, 320, 1.0 / 320
     cons = _rough_periodic_state(ni, nj, h, seed=3)
