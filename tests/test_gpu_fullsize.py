@@ -217,7 +217,7 @@ def test_rt_reflecting_sides_1024x4096_mirror_symmetry(oracle):
 
 @pytest.mark.timeout(900)
 @pytest.mark.skipif(os.environ.get("DENSE_LONG") != "1",
-                    reason="full run to t = 0.2 (about 1,650 steps; the reference takes minutes on the host cores); "
+                    reason="full run to t = 0.2 (2,422 steps; the reference takes minutes on the host cores); "
                            "set DENSE_LONG=1")
 def test_dmr_960x240_to_t_end_vs_reference(oracle):
     """The double Mach workload as configured, run to t = 0.2 with the
