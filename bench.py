@@ -60,6 +60,13 @@ FLOPS_PER_CELL_UPDATE = 6063.015625
 # State traffic of a fused ping-pong RK3 step: 4 FP64 variables,
 # read U^n / write U1; read U^n,U1 / write U2; read U^n,U2 / write U^n+1.
 BYTES_PER_CELL_UPDATE = 256.0
+# The same count for every switch pair on double Mach 1920x480 (first step from
+# the initial condition; AdaWENO's depends on how many faces take the
+# characteristic branch): python oracle/count_ops.py 5 1920 480 <splitting> <recon>
+SCHEME_FLOPS = {(101, 251): 794.21875, (101, 253): 3677.96875, (101, 255): 4014.4,
+                (102, 251): 788.2, (102, 253): 3671.90625, (102, 255): 4008.34375,
+                (103, 251): 1412.59375, (103, 253): 4296.34375, (103, 255): 4632.8,
+                (104, 255): FLOPS_PER_CELL_UPDATE, (105, 255): 2026.7}
 PEAK_FP64_VALU = 78.6e12
 PEAK_HBM = 8.0e12
 
@@ -149,6 +156,41 @@ def workloads_leg(reps: int = 3):
                         cell_updates_per_s=cells * steps / best[0], gpu_cell_updates_per_s=cells * steps / (best[1] * 1e-3),
                         ms_per_step=best[0] / steps * 1e3))
     return out
+
+
+def scheme_table(steps: int = 160):
+    """Every switch pair on double Mach 1920x480 from the initial condition with
+    the automatic tile shape: per pair, us per stage launch (HIP events on the
+    library's stream), cell-updates/s (wall clock over graph-replayed steps, best
+    of two timed runs after one that captures the graphs), and the algorithmic
+    FP64 and HBM rates of the stage kernel against MI355X's peaks."""
+    D = importlib.import_module("2dense_amd")
+    cons = dmr_initial_state(NI, NJ)
+    rows = []
+    for (sp, rc), flops in SCHEME_FLOPS.items():
+        best = None
+        with D.Solver(D.make_cfg(NI, NJ, benchmark=5, splitting=sp, recon=rc, cfl=CFL)) as s:
+            for r in range(3):
+                s.set_state(cons)
+                s.current_time, s.current_step = 0.0, 0
+                s.sync()
+                t0 = time.perf_counter()
+                n = s.run(steps)
+                s.sync()
+                wall = time.perf_counter() - t0
+                if n != steps:
+                    raise RuntimeError(f"{sp}/{rc}: ran {n} of {steps} steps")
+                ms, launches = s.last_run_gpu_ms()
+                if r > 0 and (best is None or wall < best[0]):
+                    best = (wall, ms / launches * 1e-3)
+        wall, t_launch = best
+        cells_launch = NI * NJ
+        tflops = flops / 3.0 * cells_launch / t_launch
+        gbs = BYTES_PER_CELL_UPDATE / 3.0 * cells_launch / t_launch
+        rows.append(dict(splitting=sp, recon=rc, us_per_launch=t_launch * 1e6, cell_updates_per_s=cells_launch * steps / wall,
+                         flops_per_cell_update=flops, algorithmic_tflops=tflops / 1e12, fp64_frac=tflops / PEAK_FP64_VALU,
+                         algorithmic_hbm_gbs=gbs / 1e9, hbm_frac=gbs / PEAK_HBM))
+    return rows
 
 
 def gpu_leg(steps: int, warmup: int):
@@ -353,11 +395,17 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-workloads", action="store_true")
+    ap.add_argument("--scheme-table", action="store_true",
+                    help="only time every switch pair on double Mach 1920x480 (one JSON line per pair)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run one plain device-to-device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.scheme_table:
+        for row in scheme_table():
+            print(json.dumps(row), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
