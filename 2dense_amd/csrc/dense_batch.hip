@@ -93,6 +93,7 @@ batch_stage_kernel(BatchArgs b)
     __syncthreads();
     if (tid == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        wait_vm_writes();
         const unsigned int prev =
             __hip_atomic_fetch_add(&bs->blocks_done.v, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_last = (prev == gridDim.x - 1);

@@ -68,6 +68,11 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     return v;
 }
 
+// s_waitcnt vmcnt(0): every vector-memory operation of this wave, stores and
+// atomics without return included, has been acknowledged by the memory
+// system (expcnt and lgkmcnt left unconstrained: gfx9 encoding).
+__device__ __forceinline__ void wait_vm_writes() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // Block-wide max of (ei, ej) bit patterns into the out slot.  Launches of up
 // to 512 workgroups apply both atomics at once: reading the slot first is a
 // memory round trip that each workgroup would wait for at its end (measured
@@ -109,9 +114,11 @@ __device__ __forceinline__ void block_max_to_slot(unsigned long long ei, unsigne
 // The last workgroup of a launch advances the kernel sequence number, clears
 // the slot the next launch reduces into, checks the new wave speeds and, at
 // the end of a time step, advances the device clock (general_main.f90:35-37).
-// Only atomics cross workgroups here: each block's max has been applied
-// (agent-scope atomic, waited for by the workgroup fence) before its counter
-// increment, so the last block reads final maxima.  Stores to the state
+// Only atomics cross workgroups here: each block waits until its max
+// atomics have been performed (wait_vm_writes) before its counter increment,
+// so the last block reads final maxima.  (A workgroup-scope fence does not
+// wait for global operations on this target, and an agent-scope release
+// fence would write the L2 back.)  Stores to the state
 // arrays and the DevState fields reach the next launch through the kernel
 // boundary, so no device-wide fence (an L2 write-back per block on a
 // multi-XCD part) is needed.
@@ -120,6 +127,7 @@ __device__ void launch_epilogue(DevState* st, int seq, bool end_of_step, double 
     __syncthreads();
     if (threadIdx.x != 0) return;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    wait_vm_writes();
     const unsigned int prev =
         __hip_atomic_fetch_add(&st->blocks_done.v, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (prev != gridDim.x - 1) return;
