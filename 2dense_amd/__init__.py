@@ -126,8 +126,9 @@ def make_cfg(ni: int, nj: int, *, benchmark: int = 0, splitting: int = LLF, reco
     init_RT_general.f90:9-10 does); for benchmark 0 give dx, dy, bc (4 kinds)
     and bc_state (4 x (rho,u,v,p)).  gamma=None keeps the problem's value (1.4
     unless the problem sets one).  tile: 0 is the automatic shape (16x15),
-    1-3 force 32x7, 8x7, 32x15 and 4 forces 16x15 (Solver.autotune picks one
-    by timing); results do not depend on it."""
+    1-3 force 32x7, 8x7, 32x15, 4 forces 16x15 and 5 16x15 with 512 threads
+    (x and y faces on separate halves); Solver.autotune picks one by timing;
+    results do not depend on it."""
     lib = load_library()
     c = Cfg()
     c.gamma = 1.4

@@ -201,6 +201,9 @@ static hipError_t launch_batch_one(int grid_pct, const BatchArgs& b, hipStream_t
 template <class T>
 static hipError_t launch_batch_scheme(int scheme, int grid_pct, const BatchArgs& b, hipStream_t s)
 {
+    if constexpr (T::DIRSPLIT) {
+        return hipErrorInvalidValue;  // not a batch schedule (dense_batch_set_schedule takes 0..3)
+    } else {
     switch (scheme) {
     case S_LLF_UP5: return launch_batch_one<S_LLF_UP5, T>(grid_pct, b, s);
     case S_LLF_JS: return launch_batch_one<S_LLF_JS, T>(grid_pct, b, s);
@@ -214,6 +217,7 @@ static hipError_t launch_batch_scheme(int scheme, int grid_pct, const BatchArgs&
     case S_CH: return launch_batch_one<S_CH, T>(grid_pct, b, s);
     case S_SC: return launch_batch_one<S_SC, T>(grid_pct, b, s);
     default: return hipErrorInvalidValue;
+    }
     }
 }
 
@@ -417,7 +421,8 @@ extern "C" int dense_batch_create(const dense_cfg* cfgs, int n, dense_batch_hand
         }
     }
     if (b->stages < 2) return bail(bfail(nullptr, DENSE_ERR_UNSUPPORTED, "batched mode needs stages >= 2"));
-    b->tile = (cfgs[0].tile == 4) ? 0 : cfgs[0].tile;  // the batch's own schedule: 0 is the 16x15 shape
+    // the batch's own schedule (shapes 0..3, 0 = 16x15); cfg.tile 5 maps to 16x15 too
+    b->tile = (cfgs[0].tile == 4 || cfgs[0].tile == 5) ? 0 : cfgs[0].tile;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
         return bail(bfail(nullptr, DENSE_ERR_HIP, "stream/event creation failed"));

@@ -79,7 +79,7 @@ int validate_cfg(const dense_cfg* c, char* why, size_t n)
     if (scheme_of(c->splitting, c->recon) < 0)
         return snprintf(why, n, "unsupported switch pair splitting=%d recon=%d", c->splitting, c->recon),
                DENSE_ERR_UNSUPPORTED;
-    if (c->tile < 0 || c->tile > 4) return snprintf(why, n, "tile index out of range"), DENSE_ERR_ARG;
+    if (c->tile < 0 || c->tile > 5) return snprintf(why, n, "tile index out of range"), DENSE_ERR_ARG;
     for (int s = 0; s < 4; s++)
         if (c->bc[s] < 0 || c->bc[s] > 6) return snprintf(why, n, "bad boundary kind on side %d", s), DENSE_ERR_ARG;
     for (int s = 1; s < 4; s++)
@@ -589,7 +589,7 @@ extern "C" int dense_autotune(dense_handle h, int trial_steps, int* tile, double
     if (hipMemcpyAsync(save, h->U[0], bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess)
         rc = fail(h, DENSE_ERR_HIP, "state snapshot failed");
-    for (int t : {4, 3, 1, 2}) {  // 16x15, 32x15, 32x7, 8x7
+    for (int t : {4, 3, 1, 2, 5}) {  // 16x15, 32x15, 32x7, 8x7, 16x15 with a thread per face
         if (rc) break;
         use_tile(t);
         for (int pass = 0; pass < 2 && !rc; pass++) {  // the first pass builds the graph

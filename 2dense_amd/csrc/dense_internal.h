@@ -132,10 +132,12 @@ int tile_count(int tile, int ni, int nj, int* tiles_x);
 // grids (32x15 is 4-12 % faster there); the 32x15 tile is up to 31 % slower on
 // large grids (two 256-thread workgroups per CU hide each other's load and
 // barrier phases; one 512-thread workgroup cannot).  dense_autotune times the
-// shapes for a given grid.  1..3 force those shapes and 4 forces 16x15.
+// shapes for a given grid.  1..3 force those shapes, 4 forces 16x15 and 5 the
+// 16x15 tile with 512 threads, one face per thread (shape 4 of the list:
+// faster by a few % on some grids of 40,000-230,000 cells, slower on large ones).
 inline int resolve_tile(int tile, int /*splitting*/, long long /*cells*/)
 {
-    return tile == 4 ? 0 : tile;
+    return tile == 4 ? 0 : tile == 5 ? 4 : tile;
 }
 
 }  // namespace dense

@@ -24,6 +24,9 @@ struct Tile {
     static constexpr int NXF = (TX + 1) * TY, NYF = TX * (TY + 1);
     static_assert(NXF <= NT && NYF <= NT, "one face per thread per direction");
     static_assert(4 * (NXF + NYF) <= REC_N * NC, "h arrays alias the cell record");
+    // DIRSPLIT: threads [0, NT/2) take the x faces and [NT/2, NT) the y faces
+    // (one face per thread instead of one per direction).
+    static constexpr bool DIRSPLIT = NXF <= NT / 2 && NYF <= NT / 2;
 };
 
 // Tile shapes (index = dense_cfg.tile).  Chosen so that each direction has
@@ -32,9 +35,10 @@ struct Tile {
     X(0, 16, 15, 256)      \
     X(1, 32, 7, 256)       \
     X(2, 8, 7, 64)         \
-    X(3, 32, 15, 512)
+    X(3, 32, 15, 512)      \
+    X(4, 16, 15, 512)
 
-constexpr int NTILES_SHAPES = 4;
+constexpr int NTILES_SHAPES = 5;
 
 __host__ __device__ inline void tile_shape(int tile, int& tx, int& ty, int& nt)
 {
@@ -249,6 +253,26 @@ __device__ __forceinline__ void run_tile(const StageArgs& a, int tile, double ai
     }
 
     // ---- 2. faces: x faces (fx in [0,TX], ly in [0,TY)), then y faces
+    if constexpr (T::DIRSPLIT) {
+        if (tid < NT / 2) {
+            const int fx = tid % (TX + 1), lyx = tid / (TX + 1);
+            double hx[8];
+            if (tid < T::NXF && x0 + fx <= ni && y0 + lyx < nj) {
+                eval_face<SCHEME, 0, NC, 1>(s, (lyx + 3) * W + fx, a, ai, hx);
+#pragma unroll
+                for (int k = 0; k < 4; k++) shx[k * T::NXF + lyx * (TX + 1) + fx] = hx[k];
+            }
+        } else {
+            const int f = tid - NT / 2;
+            const int lxy = f % TX, fy = f / TX;
+            double hy[8];
+            if (f < T::NYF && x0 + lxy < ni && y0 + fy <= nj) {
+                eval_face<SCHEME, 1, NC, W>(s, fy * W + lxy + 3, a, aj, hy);
+#pragma unroll
+                for (int k = 0; k < 4; k++) shy[k * T::NYF + fy * TX + lxy] = hy[k];
+            }
+        }
+    } else {
     {
         const int fx = tid % (TX + 1), lyx = tid / (TX + 1);
         double hx[8];
@@ -268,6 +292,7 @@ __device__ __forceinline__ void run_tile(const StageArgs& a, int tile, double ai
 #pragma unroll
             for (int k = 0; k < 4; k++) shy[k * T::NYF + fy * TX + lxy] = hy[k];
         }
+    }
     }
     __syncthreads();
 

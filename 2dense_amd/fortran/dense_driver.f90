@@ -56,7 +56,7 @@ module driver_state
 contains
 
   ! Whether to pick the stage kernel's tile shape by timing before the run
-  ! (dense_autotune: every shape gives the same bits).  Worth its 128 trial
+  ! (dense_autotune: every shape gives the same bits).  Worth its 160 trial
   ! steps on long runs only; DENSE_AUTOTUNE=1 or 0 forces it on or off.
   logical function want_autotune(steps_left)
     integer, intent(in) :: steps_left

@@ -111,7 +111,7 @@ def rt_initial_state(ni: int, nj: int, gamma: float = 5.0 / 3.0) -> np.ndarray:
     return np.stack([rho, np.zeros_like(rho), rho * v, p / (gamma - 1.0) + 0.5 * rho * v * v])
 
 
-TILE_NAMES = {1: "32x7", 2: "8x7", 3: "32x15", 4: "16x15"}
+TILE_NAMES = {1: "32x7", 2: "8x7", 3: "32x15", 4: "16x15", 5: "16x15/512"}
 
 
 def workloads_leg(reps: int = 3):

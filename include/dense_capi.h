@@ -67,8 +67,9 @@ typedef struct dense_cfg {
     int bc[4];               /* per-side kind (benchmark 0) */
     double bc_state[4][4];   /* per-side Dirichlet (rho,u,v,p); DMR sides: post-shock state */
     int tile;                /* tile shape: 0 automatic (16x15), 1 32x7, 2 8x7, 3 32x15,
-                                4 16x15; dense_autotune picks one by timing.  Results do
-                                not depend on it. */
+                                4 16x15, 5 16x15 with 512 threads (x and y faces on separate
+                                halves); dense_autotune picks one by timing.  Results do not
+                                depend on it. */
 } dense_cfg;
 
 typedef struct dense_solver* dense_handle;
@@ -131,7 +132,7 @@ int dense_max_eig(dense_handle h, double out[2]);
  * at every save. */
 int dense_update_params(dense_handle h, const dense_cfg* cfg);
 
-/* Time each tile shape (cfg.tile 4, 3, 1, 2) on trial_steps steps of the
+/* Time each tile shape (cfg.tile 4, 3, 1, 2, 5) on trial_steps steps of the
  * current state, restoring state and clock after every trial, and keep the
  * fastest as an explicit cfg.tile (dense_update_params keeps it).  *tile and
  * *best_ms (nullable) receive the choice and its trial time.  No counterpart in

@@ -46,7 +46,7 @@ def test_gpu_matches_reference_golden(name, oracle):
     assert s.current_time == float(g["time"])
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("name", ["dmr_104", "ivc_101_253", "rt_105", "sv_101_255"])
 def test_gpu_tile_shape_invariance(name, tile, oracle):
     """Every tile shape gives the same bits (faces are independent; the only

@@ -138,7 +138,7 @@ def test_autotune_restores_state_and_clock_and_keeps_the_bits(splitting):
         t0, s0 = a.current_time, a.current_step
         eig0 = a.max_eig()
         tile, ms = a.autotune(trial_steps=3)
-        assert tile in (1, 2, 3, 4) and ms > 0
+        assert tile in (1, 2, 3, 4, 5) and ms > 0
         assert a.cfg.tile == tile
         assert np.array_equal(a.interior(), before)
         assert a.max_eig() == eig0
