@@ -94,12 +94,13 @@ def load_library(path: str | None = None) -> C.CDLL:
     lib.dense_last_run_gpu_ms.argtypes = [H, dp, C.POINTER(C.c_int)]
     lib.dense_update_params.argtypes = [H, P]
     lib.dense_autotune.argtypes = [H, C.c_int, C.POINTER(C.c_int), dp]
+    lib.dense_get_theta.argtypes = [H, dp]
     lib.dense_sync.argtypes = [H]
     lib.dense_last_error.argtypes = [H]
     lib.dense_last_error.restype = C.c_char_p
     for name in ("dense_problem_defaults", "dense_create", "dense_destroy", "dense_set_state",
                  "dense_get_state", "dense_step", "dense_run", "dense_max_eig", "dense_last_run_gpu_ms", "dense_update_params",
-                 "dense_autotune", "dense_sync"):
+                 "dense_autotune", "dense_get_theta", "dense_sync"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -265,6 +266,18 @@ class Solver:
         _check(self.lib, self.h, self.lib.dense_autotune(self.h, trial_steps, C.byref(t), C.byref(ms)))
         self.cfg.tile = t.value
         return t.value, ms.value
+
+    def theta(self):
+        """AdaWENO's indicator at every face (splitting 105, stages 2-3, after a
+        step), as the reference's theta arrays hold it at output time: xp, xn of
+        shape (nj, ni+1) and ep, en of shape (ni, nj+1) -- the Fortran arrays
+        theta_xi(ni+1, nj) and theta_eta(nj+1, ni) seen in row-major order."""
+        ni, nj = self.cfg.ni, self.cfg.nj
+        nx, ny = (ni + 1) * nj, (nj + 1) * ni
+        buf = np.empty(2 * nx + 2 * ny)
+        _check(self.lib, self.h, self.lib.dense_get_theta(self.h, _dp(buf)))
+        return (buf[:nx].reshape(nj, ni + 1), buf[nx:2 * nx].reshape(nj, ni + 1),
+                buf[2 * nx:2 * nx + ny].reshape(ni, nj + 1), buf[2 * nx + ny:].reshape(ni, nj + 1))
 
     def sync(self):
         _check(self.lib, self.h, self.lib.dense_sync(self.h))

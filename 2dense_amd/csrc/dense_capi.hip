@@ -25,6 +25,8 @@ struct dense_solver {
     double* dt_hist = nullptr;    // device
     int hist_cap = 0;
     double* pad_dev = nullptr;    // padded cons + prim for get_state
+    double* theta_dev = nullptr;  // AdaWENO indicator at the faces, for get_theta
+    bool stepped = false;         // a step ran since the state or the parameters were set
     double* pad_host = nullptr;   // pinned staging
     double* stage_host = nullptr; // pinned interior staging for set_state
     hipGraphExec_t graph = nullptr;
@@ -311,6 +313,7 @@ extern "C" int dense_destroy(dense_handle h)
     if (h->st_host) (void)hipHostFree(h->st_host);
     if (h->dt_hist) (void)hipFree(h->dt_hist);
     if (h->pad_dev) (void)hipFree(h->pad_dev);
+    if (h->theta_dev) (void)hipFree(h->theta_dev);
     if (h->pad_host) (void)hipHostFree(h->pad_host);
     if (h->stage_host) (void)hipHostFree(h->stage_host);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -344,6 +347,7 @@ extern "C" int dense_set_state(dense_handle h, const double* cons)
             for (int i = ni; i < h->ld; i++) dst[i] = 0.0;
         }
     HIPCHK(h, hipMemcpyAsync(h->U[0], h->stage_host, sizeof(double) * 4 * h->plane, hipMemcpyHostToDevice, h->stream));
+    h->stepped = false;
     // fresh reduction ring, then the wave speeds of the uploaded state
     int rc = read_state(h);
     if (rc) return rc;
@@ -408,6 +412,7 @@ extern "C" int dense_step(dense_handle h, double current_time, double* dt)
     if (rc) return rc;
     rc = read_state(h);
     if (rc) return rc;
+    h->stepped = true;
     if (dt) *dt = h->st_host->dt;
     if (h->st_host->error) return fail(h, DENSE_ERR_NONFINITE, "non-finite wave speed after step");
     return DENSE_OK;
@@ -492,6 +497,7 @@ extern "C" int dense_run(dense_handle h, int max_steps, double total_time, int t
     }
     *t = s.t;
     *step = s.step;
+    if (taken > 0) h->stepped = true;
     if (steps_taken) *steps_taken = taken;
     float ms = 0.f;
     HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
@@ -542,6 +548,7 @@ extern "C" int dense_update_params(dense_handle h, const dense_cfg* cfg)
     h->tile = resolve_tile(c.tile, c.splitting, (long long)c.ni * c.nj);
     h->ntiles = tile_count(h->tile, c.ni, c.nj, &h->tiles_x);
     release_graph(h);  // kernel arguments changed
+    h->stepped = false;  // the wave-speed ring restarts below
     // the wave speeds of the current state depend on gamma
     int r2 = read_state(h);
     if (r2) return r2;
@@ -568,6 +575,7 @@ extern "C" int dense_autotune(dense_handle h, int trial_steps, int* tile, double
     int rc = read_state(h);
     if (rc) return rc;
     const DevState st0 = *h->st_host;
+    const bool stepped0 = h->stepped;
     const size_t bytes = sizeof(double) * 4 * h->plane;
     double* save = nullptr;
     HIPCHK(h, hipMalloc(&save, bytes));
@@ -604,11 +612,49 @@ extern "C" int dense_autotune(dense_handle h, int trial_steps, int* tile, double
     }
     use_tile(rc ? keep : bt);
     const int rr = restore();
+    h->stepped = stepped0;
     (void)hipFree(save);
     if (rc) return rc;
     if (rr) return rr;
     if (tile) *tile = bt;
     if (best_ms) *best_ms = best;
+    return DENSE_OK;
+}
+
+// AdaWENO's indicator theta at every face, as the reference's theta arrays hold
+// it at output time: from the last flux evaluation, i.e. the input state of the
+// last RK stage (U^(2) of a 3-stage step, U^(1) of a 2-stage one) with its
+// boundary values at t^n and its global wave speeds.  Fills theta_xip(ni+1,nj),
+// theta_xin(ni+1,nj), theta_etap(nj+1,ni), theta_etan(nj+1,ni), Fortran order.
+extern "C" int dense_get_theta(dense_handle h, double* theta)
+{
+    if (!h || !theta) return fail(h, DENSE_ERR_ARG, "null argument");
+    if (h->cfg.splitting != 105)
+        return fail(h, DENSE_ERR_UNSUPPORTED, "theta is the indicator of AdaWENO (splitting 105)");
+    if (h->cfg.stages < 2)
+        return fail(h, DENSE_ERR_UNSUPPORTED, "theta needs the input state of the last stage (stages 2 or 3)");
+    if (!h->stepped) return fail(h, DENSE_ERR_ARG, "no step since the state or the parameters were set");
+    const int ni = h->cfg.ni, nj = h->cfg.nj;
+    const size_t n = 2 * ((size_t)(ni + 1) * nj + (size_t)(nj + 1) * ni);
+    if (!h->theta_dev && hipMalloc(&h->theta_dev, sizeof(double) * n) != hipSuccess)
+        return fail(h, DENSE_ERR_NOMEM, "theta buffer allocation failed");
+    int rc = read_state(h);
+    if (rc) return rc;
+    const DevState s = *h->st_host;
+    const int slot = (s.seq - 1) & 3;  // read by the last stage launch
+    double ai, aj;
+    std::memcpy(&ai, &s.alpha[slot][0], sizeof(double));
+    std::memcpy(&aj, &s.alpha[slot][1], sizeof(double));
+    if (h->bc_rows) {
+        // the inflow ghost cells at t^n (a get_state since may have set them for t^(n+1))
+        ssl_clock_terms(s.t_prev, h->bct_host);
+        HIPCHK(h, hipMemcpyAsync(&h->st->bc_t[0], h->bct_host, sizeof(double) * 2, hipMemcpyHostToDevice, h->stream));
+    }
+    StageArgs a = base_args(h);
+    a.us = h->U[h->cfg.stages - 1];
+    HIPCHK(h, launch_theta(a, ai, aj, s.t_prev, h->theta_dev, h->stream));
+    HIPCHK(h, hipMemcpyAsync(theta, h->theta_dev, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     return DENSE_OK;
 }
 

@@ -46,6 +46,7 @@ struct alignas(128) DevState {
     alignas(128) double t;           // t^n (the clock words: one line, read by every workgroup)
     double dt;                       // step size of the current step
     double t_end;                    // run while t < t_end ...
+    double t_prev;                   // t^n of the last completed step (its stages' boundary time)
     int step;                        // ... and step < step_end
     int step_end;
     int seq;                         // kernel sequence number: reads alpha[seq&3], writes alpha[(seq+1)&3]
@@ -118,6 +119,9 @@ hipError_t launch_batch_stage(int scheme, int tile, int grid_pct, const BatchArg
 hipError_t launch_batch_max(const BatchArgs& b, hipStream_t s);
 
 // host-side launchers (dense_kernels.hip)
+// AdaWENO's indicator theta at every x and y face of the state a.us (boundary
+// time t, global wave speeds ai, aj) into the reference's theta arrays.
+hipError_t launch_theta(const StageArgs& a, double ai, double aj, double t, double* theta, hipStream_t s);
 hipError_t launch_stage(int scheme, int tile, const StageArgs& a, hipStream_t s);
 hipError_t launch_max(const StageArgs& a, hipStream_t s);
 hipError_t launch_set_clock(DevState* st, double t, double t_end, int step, int step_end, hipStream_t s,

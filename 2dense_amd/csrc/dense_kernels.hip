@@ -74,6 +74,45 @@ __global__ void __launch_bounds__(256) max_kernel(StageArgs a)
     launch_epilogue(st, seq, false, 0.0, 0.0, nullptr);
 }
 
+// AdaWENO's indicator theta at the faces of one sweep direction: the WENO-Z
+// smoothness measure of rho + F+-_n over each face's two 5-point stencils
+// (get_theta_xi_p/_n and get_theta_eta_p/_n, scheme_weno5th_SC.f90:1-64 and
+// the eta analogues, on the indicator of flux_splitting_SC.f90:58-59, :89-90),
+// the same values the stage kernel's AdaWENO faces branch on.  Ghost cells from
+// the boundary rules at time t.  Output as the reference's theta arrays, first
+// index fastest: DIR 0 theta_xi(ni+1, nj), DIR 1 theta_eta(nj+1, ni); + then -.
+template <int DIR>
+__global__ void __launch_bounds__(256) theta_kernel(StageArgs a, double alpha, double t, double* thp, double* thn)
+{
+    constexpr int NRM = DIR == 0 ? 1 : 2;
+    const int nf = DIR == 0 ? a.ni + 1 : a.nj + 1, nl = DIR == 0 ? a.nj : a.ni;
+    const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (long long)nf * nl) return;
+    const int f = (int)(idx % nf), l = (int)(idx / nf);
+    double rho[6], mn[6], un[6], p[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        const int g = f - 3 + m;  // cell along the sweep: f-3 .. f+2
+        const int gx = DIR == 0 ? g : l, gy = DIR == 0 ? l : g;
+        double c[4], pr[4];
+        if ((unsigned)g < (unsigned)(DIR == 0 ? a.ni : a.nj)) load_interior(a, a.us, gx, gy, c, pr);
+        else ghost_cell(a, a.us, gx, gy, t, c, pr);
+        rho[m] = c[0];
+        mn[m] = c[NRM];
+        un[m] = pr[DIR];
+        p[m] = pr[2];
+    }
+    double lf[5], w0, w1, w2, th;
+#pragma unroll
+    for (int i = 0; i < 5; i++) lf[i] = rho[i] + 0.5 * ((un[i] + alpha) * mn[i] + p[i]);
+    weno5z_omega<false>(lf, w0, w1, w2, &th);
+    thp[idx] = th;
+#pragma unroll
+    for (int i = 0; i < 5; i++) lf[i] = rho[5 - i] + 0.5 * ((un[5 - i] - alpha) * mn[5 - i] + p[5 - i]);
+    weno5z_omega<false>(lf, w0, w1, w2, &th);
+    thn[idx] = th;
+}
+
 __global__ void set_clock_kernel(DevState* st, double t, double t_end, int step, int step_end, double bc_t0,
                                  double bc_t1)
 {
@@ -181,6 +220,18 @@ hipError_t launch_stage(int scheme, int tile, const StageArgs& a, hipStream_t s)
 #undef X
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_theta(const StageArgs& a, double ai, double aj, double t, double* theta, hipStream_t s)
+{
+    const long long nx = (long long)(a.ni + 1) * a.nj, ny = (long long)(a.nj + 1) * a.ni;
+    hipLaunchKernelGGL(theta_kernel<0>, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, s, a, ai, t, theta,
+                       theta + nx);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(theta_kernel<1>, dim3((unsigned)((ny + 255) / 256)), dim3(256), 0, s, a, aj, t,
+                       theta + 2 * nx, theta + 2 * nx + ny);
+    return hipGetLastError();
 }
 
 hipError_t launch_max(const StageArgs& a, hipStream_t s)

@@ -149,6 +149,7 @@ __device__ void launch_epilogue(DevState* st, int seq, bool end_of_step, double 
     if (end_of_step) {
         if (dt_hist && st->hist_n < st->hist_cap) dt_hist[st->hist_n] = dt;
         st->hist_n += 1;
+        st->t_prev = t;
         st->t = t + dt;
         st->step += 1;
         st->active = (st->t < st->t_end) && (st->step < st->step_end) && !st->error;

@@ -71,6 +71,12 @@ module dense_capi
       real(c_double), intent(out) :: best_ms
     end function
 
+    integer(c_int) function dense_get_theta(h, theta) bind(C, name="dense_get_theta")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: theta(*)
+    end function
+
     type(c_ptr) function dense_last_error(h) bind(C, name="dense_last_error")
       import :: c_ptr
       type(c_ptr), value :: h
@@ -100,5 +106,20 @@ contains
     write(*, '(a,a,i0,a,*(a))') what, ' failed (', rc, '): ', chars
     stop 1
   end subroutine dense_check
+
+  ! Print the library's message for a failed call and carry on.
+  subroutine dense_warn(rc, h, what)
+    integer(c_int), intent(in) :: rc
+    type(c_ptr), intent(in) :: h
+    character(len=*), intent(in) :: what
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: chars(:)
+    integer :: n
+    if (rc == DENSE_OK) return
+    p = dense_last_error(h)
+    n = int(c_strlen(p))
+    call c_f_pointer(p, chars, [n])
+    write(*, '(a,a,i0,a,*(a))') what, ' failed (', rc, '): ', chars
+  end subroutine dense_warn
 
 end module dense_capi

@@ -415,6 +415,49 @@ contains
     close(10)
   end subroutine write_tec
 
+  ! 2d_theta_<step>.dat (output_test_general.f90:61-119): per cell the larger of
+  ! AdaWENO's indicator at its two faces, for each sweep and sign.  th holds
+  ! theta_xip(ni+1,nj), theta_xin, theta_etap(nj+1,ni), theta_etan.
+  subroutine write_theta(th)
+    double precision, intent(in) :: th(*)
+    character(len=10) :: temp
+    integer :: i, j, k, nx, ny, o
+    nx = (ni + 1) * nj
+    ny = (nj + 1) * ni
+    write(temp, '(i8)') current_step
+    open(10, file='2d_theta_' // trim(adjustl(temp)) // '.dat', form='formatted')
+    write(10, *) 'title=', '"flow time ', current_time, '"'
+    write(10, *) 'variables="x","y","xp","xn","ep","en"'
+    write(10, *) 'ZONE I=', ni + 1, ' J=', nj + 1, 'DATAPACKING=BLOCK, VARLOCATION=([3-6]=CELLCENTERED)'
+    do j = 1, nj + 1
+      do i = 1, ni + 1
+        write(10, *) vet_x(i, j)
+      end do
+    end do
+    do j = 1, nj + 1
+      do i = 1, ni + 1
+        write(10, *) vet_y(i, j)
+      end do
+    end do
+    do k = 0, 1
+      o = k * nx
+      do j = 1, nj
+        do i = 1, ni
+          write(10, *) max(th(o + i + (j - 1) * (ni + 1)), th(o + i + 1 + (j - 1) * (ni + 1)))
+        end do
+      end do
+    end do
+    do k = 0, 1
+      o = 2 * nx + k * ny
+      do j = 1, nj
+        do i = 1, ni
+          write(10, *) max(th(o + j + (i - 1) * (nj + 1)), th(o + j + 1 + (i - 1) * (nj + 1)))
+        end do
+      end do
+    end do
+    close(10)
+  end subroutine write_theta
+
 end module driver_state
 
 
@@ -430,6 +473,8 @@ program dense_gpu
   double precision :: cpu_t, step_dt
   integer(c_int) :: tile_choice
   real(c_double) :: trial_ms
+  double precision, allocatable :: theta_buf(:)
+  integer :: theta_step = -1
 
   call read_input()
   ni = single_ni
@@ -446,6 +491,7 @@ program dense_gpu
     call load_binary()
   end if
 
+  allocate(theta_buf(2 * ((ni + 1) * nj + (nj + 1) * ni)))
   call fill_cfg(cfg)
   rc = dense_create(cfg, h)
   call dense_check(rc, c_null_ptr, 'dense_create')
@@ -482,6 +528,7 @@ program dense_gpu
       if (if_save_tec) then
         print *, 'Saving TEC data file at time = ', current_time
         call write_tec()
+        if (if_save_theta) call save_theta()
         elasp_time = 0.d0
       end if
       print *, 'Saving finished!'
@@ -499,6 +546,24 @@ program dense_gpu
   rc = dense_get_state(h, cons, c_loc(prim))
   call dense_check(rc, h, 'dense_get_state')
   call write_tec()
+  if (if_save_theta) call save_theta()
   rc = dense_destroy(h)
   print *, 'Calculation is completed! '
+contains
+
+  ! the reference writes the theta file beside every Tecplot file when
+  ! IF_SAVE_THETA is set (output_test_general.f90:10-12); its final write after
+  ! the loop repeats the file of a save at the same step, which stays as written
+  ! (the library's indicator state restarts with the NAMELIST re-read)
+  subroutine save_theta()
+    integer(c_int) :: r
+    if (current_step == theta_step) return
+    r = dense_get_theta(h, theta_buf)
+    if (r == DENSE_OK) then
+      call write_theta(theta_buf)
+      theta_step = current_step
+    else
+      call dense_warn(r, h, 'theta file')
+    end if
+  end subroutine save_theta
 end program dense_gpu

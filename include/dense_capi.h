@@ -139,6 +139,15 @@ int dense_update_params(dense_handle h, const dense_cfg* cfg);
  * the reference (one CPU code path); all shapes give the same bits. */
 int dense_autotune(dense_handle h, int trial_steps, int* tile, double* best_ms);
 
+/* AdaWENO's indicator theta (splitting 105, stages 2 or 3, after at least one
+ * step) at every face, as the reference's theta arrays hold it when it writes
+ * 2d_theta_<step>.dat (output_test_general.f90:61-119): from the last stage's
+ * flux evaluation (get_theta_xi_p/_n, get_theta_eta_p/_n,
+ * scheme_weno5th_SC.f90:1-64 and the eta analogues).  theta receives
+ * theta_xip(ni+1,nj), theta_xin(ni+1,nj), theta_etap(nj+1,ni),
+ * theta_etan(nj+1,ni), Fortran order, one after the other. */
+int dense_get_theta(dense_handle h, double* theta);
+
 /* Wait for queued device work of this handle. */
 int dense_sync(dense_handle h);
 

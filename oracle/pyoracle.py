@@ -212,7 +212,8 @@ def write_input_in(path: str, *, benchmark: int, ni: int, nj: int, splitting: in
                    cfl: float, total_step: int, save_step: int, total_time: float = 1.0e30,
                    gamma: float = 1.4, delta: float = 0.5, threads: int = 1, stages: int = 3,
                    if_const_dt: bool = False, dt: float = 0.01, if_bodyforce: bool = False,
-                   if_save_binary: bool = True, if_save_tec: bool = False, dt_save: float = 1.0e30):
+                   if_save_binary: bool = True, if_save_tec: bool = False, dt_save: float = 1.0e30,
+                   if_save_theta: bool = False):
     """Six NAMELIST groups in the order read_par_namelist reads them (flow_mod_general.f90:313-332)."""
     L = lambda b: "T" if b else "F"  # noqa: E731
     txt = f"""&OMP_PARS
@@ -232,7 +233,7 @@ def write_input_in(path: str, *, benchmark: int, ni: int, nj: int, splitting: in
  /
 &SCHEME_PARS
  SCHEME_SPLITTING_INDEX={splitting}, SCHEME_INV_INDEX={recon}, SCHEME_VIS_INDEX=301,
- IF_SAVE_THETA=F, DELTA={delta!r},
+ IF_SAVE_THETA={L(if_save_theta)}, DELTA={delta!r},
  /
 &GAS_PARS
  GAMMA={gamma!r},

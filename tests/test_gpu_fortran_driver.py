@@ -85,3 +85,40 @@ def test_driver_autotune_keeps_the_bits(tmp_path, name):
     assert np.array_equal(dts_on, g["dts"])
     for d in (on, off):
         assert np.array_equal(d["cons"][:, 4:-4, 4:-4], g["cons"])
+
+
+@pytest.mark.parametrize("name", ["dmr_105", "rt_105", "sv_105", "ssl_105_30"])
+def test_driver_theta_file_matches_reference(tmp_path, name):
+    """With IF_SAVE_THETA the driver writes 2d_theta_<step>.dat beside its
+    Tecplot file, as the reference does (output_test_general.f90:10-12,
+    :61-119): AdaWENO's indicator from the last stage's flux evaluation, the
+    larger of each cell's two faces per sweep and sign.  The file equals the
+    reference binary's, text for text."""
+    import pyoracle as po
+    if not os.path.exists(DRIVER):
+        pytest.fail("Fortran driver not built (python 2dense_amd/build.py)")
+    if not os.path.exists(po.REF_BIN):
+        pytest.fail("reference binary not built (oracle/build_ref.sh)")
+    c = CASES[name]
+    steps = c["steps"]
+    print(f"theta file case {name}", flush=True)
+    nml = dict(benchmark=c["benchmark"], ni=c["ni"], nj=c["nj"], splitting=105, recon=c["recon"], cfl=c["cfl"],
+               gamma=c.get("gamma", 1.4), if_bodyforce=c.get("if_bodyforce", False), total_step=steps,
+               save_step=steps, if_save_tec=True, if_save_theta=True)
+    files = {}
+    for who, exe in (("ref", po.REF_BIN), ("drv", DRIVER)):
+        d = tmp_path / who
+        d.mkdir()
+        po.write_input_in(str(d / "input.in"), **nml)
+        out = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+        files[who] = (d / f"2d_theta_{steps}.dat").read_text().splitlines()
+    ref, got = files["ref"], files["drv"]
+    assert len(got) == len(ref) == 3 + 2 * (c["ni"] + 1) * (c["nj"] + 1) + 4 * c["ni"] * c["nj"]
+    assert got[:3] == ref[:3]
+    # vertex coordinates, then the four indicator fields: the same values
+    a = np.array([float(x.replace("D", "E")) for x in got[3:]])
+    b = np.array([float(x.replace("D", "E")) for x in ref[3:]])
+    assert np.array_equal(a, b), f"max |diff| {np.max(np.abs(a - b))}"
+    th = a[2 * (c["ni"] + 1) * (c["nj"] + 1):]
+    assert np.all((th > 0) & (th <= 1))

@@ -147,3 +147,34 @@ def test_autotune_restores_state_and_clock_and_keeps_the_bits(splitting):
         b.run(7)
         assert (a.current_time, a.current_step) == (b.current_time, b.current_step)
         assert np.array_equal(a.interior(), b.interior())
+
+
+def test_theta_is_one_on_a_uniform_state_and_rejected_elsewhere():
+    """AdaWENO's indicator theta = 1/(1 + (sum alpha - 1)^2) (weno_mod.f90:94):
+    on a uniform state the smoothness indicators are equal, tau5 = 0, the
+    WENO-Z alphas are the linear weights and theta is exactly 1 at every face.
+    dense_get_theta refuses other splittings, single-stage runs and a state
+    with no step since it was set."""
+    D = dense()
+    n = 48
+    q = np.array([1.3, 0.4 * 1.3, -0.2 * 1.3, 2.5 / 0.4 + 0.5 * 1.3 * (0.16 + 0.04)])
+    cons = np.broadcast_to(q[:, None, None], (4, n, n)).copy()
+    cfg = D.make_cfg(n, n, benchmark=0, splitting=105, recon=255, cfl=0.5, gamma=1.4, dx=1.0 / n, dy=1.0 / n,
+                     bc=[D.BC_PERIODIC] * 4)
+    with D.Solver(cfg) as s:
+        s.set_state(cons)
+        with pytest.raises(D.DenseError):
+            s.theta()  # no step yet
+        s.step(2)
+        xp, xn, ep, en = s.theta()
+        assert xp.shape == (n, n + 1) and ep.shape == (n, n + 1)
+        for a in (xp, xn, ep, en):
+            assert np.all(a == 1.0)
+    for kw in (dict(splitting=104), dict(stages=1)):
+        c2 = D.make_cfg(n, n, benchmark=0, recon=255, cfl=0.5, gamma=1.4, dx=1.0 / n, dy=1.0 / n,
+                        bc=[D.BC_PERIODIC] * 4, **{"splitting": 105, **kw})
+        with D.Solver(c2) as s:
+            s.set_state(cons)
+            s.step(1)
+            with pytest.raises(D.DenseError):
+                s.theta()
