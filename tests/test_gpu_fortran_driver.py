@@ -122,3 +122,35 @@ def test_driver_theta_file_matches_reference(tmp_path, name):
     assert np.array_equal(a, b), f"max |diff| {np.max(np.abs(a - b))}"
     th = a[2 * (c["ni"] + 1) * (c["nj"] + 1):]
     assert np.all((th > 0) & (th <= 1))
+
+
+@pytest.mark.parametrize("name", ["dmr_104", "ivc_101_253", "rt_105", "sedov_104"])
+def test_driver_tecplot_file_matches_reference(tmp_path, name):
+    """With IF_SAVE_TEC the driver's 2d_<step>.dat equals the reference's
+    (output_test_general.f90:16-58): the same header lines, including the
+    variable list that names 7 fields for 8 data blocks, and the same vertex
+    coordinates and primitive fields, value for value."""
+    import pyoracle as po
+    if not os.path.exists(DRIVER) or not os.path.exists(po.REF_BIN):
+        pytest.fail("driver or reference binary not built")
+    c = CASES[name]
+    steps = c["steps"]
+    print(f"tecplot file case {name}", flush=True)
+    nml = dict(benchmark=c["benchmark"], ni=c["ni"], nj=c["nj"], splitting=c["splitting"], recon=c["recon"],
+               cfl=c["cfl"], gamma=c.get("gamma", 1.4), if_bodyforce=c.get("if_bodyforce", False),
+               stages=c.get("stages", 3), if_const_dt=c.get("if_const_dt", False), dt=c.get("dt", 0.01),
+               total_step=steps, save_step=steps, if_save_tec=True)
+    files = {}
+    for who, exe in (("ref", po.REF_BIN), ("drv", DRIVER)):
+        d = tmp_path / who
+        d.mkdir()
+        po.write_input_in(str(d / "input.in"), **nml)
+        out = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+        files[who] = (d / f"2d_{steps}.dat").read_text().splitlines()
+    ref, got = files["ref"], files["drv"]
+    assert len(got) == len(ref) == 3 + 2 * (c["ni"] + 1) * (c["nj"] + 1) + 6 * c["ni"] * c["nj"]
+    assert got[:3] == ref[:3]
+    a = np.array([float(x.replace("D", "E")) for x in got[3:]])
+    b = np.array([float(x.replace("D", "E")) for x in ref[3:]])
+    assert np.array_equal(a, b), f"max |diff| {np.max(np.abs(a - b))}"
