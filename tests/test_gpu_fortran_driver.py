@@ -154,3 +154,36 @@ def test_driver_tecplot_file_matches_reference(tmp_path, name):
     a = np.array([float(x.replace("D", "E")) for x in got[3:]])
     b = np.array([float(x.replace("D", "E")) for x in ref[3:]])
     assert np.array_equal(a, b), f"max |diff| {np.max(np.abs(a - b))}"
+
+
+@pytest.mark.parametrize("if_save_tec", [True, False])
+def test_driver_save_cadence_matches_reference(tmp_path, if_save_tec):
+    """Saves at MOD(step, SAVE_STEP) = 0 or once the save clock reaches DT_SAVE
+    (general_main.f90:38-53), the clock reset only when the Tecplot file is
+    written (so without IF_SAVE_TEC every step after the first crossing saves):
+    the driver writes its dumps at the same steps as the reference, with the
+    same interior states and times."""
+    import pyoracle as po
+    if not os.path.exists(DRIVER) or not os.path.exists(po.REF_BIN):
+        pytest.fail("driver or reference binary not built")
+    c = CASES["dmr_104"]
+    g = golden("dmr_104")
+    steps = c["steps"]
+    nml = dict(benchmark=c["benchmark"], ni=c["ni"], nj=c["nj"], splitting=c["splitting"], recon=c["recon"],
+               cfl=c["cfl"], total_step=steps, save_step=7, dt_save=float(g["time"]) / 4.5,
+               if_save_tec=if_save_tec)
+    dumps = {}
+    for who, exe in (("ref", po.REF_BIN), ("drv", DRIVER)):
+        d = tmp_path / who
+        d.mkdir()
+        po.write_input_in(str(d / "input.in"), **nml)
+        out = subprocess.run([exe], cwd=d, capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+        dumps[who] = {int(f[3:-6]): po.read_dat2d(str(d / f)) for f in os.listdir(d)
+                      if f.startswith("2d_") and f.endswith(".dat2d")}
+    print(f"save steps: reference {sorted(dumps['ref'])}, driver {sorted(dumps['drv'])}", flush=True)
+    assert sorted(dumps["drv"]) == sorted(dumps["ref"]) and len(dumps["ref"]) >= 2
+    for k in dumps["ref"]:
+        a, b = dumps["drv"][k], dumps["ref"][k]
+        assert (a["step"], a["time"]) == (b["step"], b["time"])
+        assert np.array_equal(a["cons"][:, 4:-4, 4:-4], b["cons"][:, 4:-4, 4:-4])
