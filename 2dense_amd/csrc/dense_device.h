@@ -220,6 +220,9 @@ __device__ __forceinline__ double recon(const double* lf)
 // ---------------------------------------------------------------- cells
 
 // Primitive variables of one cell, get_primary.f90:52-61.  pr = (u, v, p, c).
+// WITH_C = false leaves out the sound speed (pr[3] = 0) for callers that
+// never read it.
+template <bool WITH_C = true>
 __device__ __forceinline__ void prim_of(double gamma, const double* c, double* pr)
 {
     const double rho = c[0];
@@ -230,7 +233,7 @@ __device__ __forceinline__ void prim_of(double gamma, const double* c, double* p
     pr[0] = u;
     pr[1] = v;
     pr[2] = p;
-    pr[3] = sqrt(gamma * p / rho);
+    pr[3] = WITH_C ? sqrt(gamma * p / rho) : 0.0;
 }
 
 // Dirichlet ghost from a primitive state, boundary_doublemach_general.f90:52-61
@@ -247,13 +250,14 @@ __device__ __forceinline__ void from_state(double gamma, double rho, double u, d
     pr[3] = sqrt(gamma * p / rho);
 }
 
+template <bool WITH_C = true>
 __device__ __forceinline__ void load_interior(const StageArgs& a, const double* __restrict__ U, int gx, int gy,
                                               double* c, double* pr)
 {
     const long long o = (long long)gy * a.ld + gx;
 #pragma unroll
     for (int k = 0; k < 4; k++) c[k] = U[k * a.plane + o];
-    prim_of(a.gamma, c, pr);
+    prim_of<WITH_C>(a.gamma, c, pr);
 }
 
 // Ghost cell (gx,gy) outside the interior in exactly one direction, k layers

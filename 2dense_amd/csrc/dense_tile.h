@@ -176,6 +176,14 @@ __device__ __forceinline__ void eval_face(const double* __restrict__ s, int base
     else face_cw_lf<DIR, true, SCHEME - S_LLF_UP5, NC, ST>(s, base, alpha, h);
 }
 
+// Only the LLF (per-cell wave speed |u_n| + c) and Steger-Warming faces read a
+// cell's sound speed; the tile loader skips it for the others.
+template <int SCHEME>
+constexpr bool faces_read_sound_speed()
+{
+    return SCHEME <= S_LLF_Z || (SCHEME >= S_SW_UP5 && SCHEME <= S_SW_Z);
+}
+
 // Minimum waves per SIMD requested from the register allocator, per scheme.
 #ifndef DENSE_WAVES_CHAR
 #define DENSE_WAVES_CHAR 2
@@ -231,7 +239,7 @@ __device__ __forceinline__ void run_tile(const StageArgs& a, int tile, double ai
         const bool inx = (unsigned)gx < (unsigned)ni, iny = (unsigned)gy < (unsigned)nj;
         if ((!inx && !iny) || gx >= ni + 3 || gy >= nj + 3) continue;
         double c[4], pr[4];
-        if (inx && iny) load_interior(a, a.us, gx, gy, c, pr);
+        if (inx && iny) load_interior<faces_read_sound_speed<SCHEME>()>(a, a.us, gx, gy, c, pr);
         else ghost_cell(a, a.us, gx, gy, t, c, pr);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
