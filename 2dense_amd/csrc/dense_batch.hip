@@ -21,10 +21,10 @@ namespace dense {
 int validate_cfg(const dense_cfg* c, char* why, size_t n);  // dense_capi.hip
 
 template <int SCHEME, class T>
-__global__ void __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(min_waves<SCHEME>())))
+__global__ void __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(min_waves<SCHEME, T>())))
 batch_stage_kernel(BatchArgs b)
 {
-    __shared__ TileLds<T> L;
+    __shared__ TileLds<T, alias_h<SCHEME, T>()> L;
     __shared__ StageArgs sa;  // the current instance, as the single-block kernel sees its block
     __shared__ double s_ai, s_aj, s_dt, s_t;
     __shared__ int s_tile, s_last;

@@ -11,10 +11,10 @@ namespace dense {
 // per-launch work (clock/state reads, the wave-speed reduction and its
 // atomics) is paid once per workgroup rather than once per tile.
 template <int SCHEME, class T>
-__global__ void __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(min_waves<SCHEME>())))
+__global__ void __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(min_waves<SCHEME, T>())))
 stage_kernel(StageArgs a)
 {
-    __shared__ TileLds<T> L;
+    __shared__ TileLds<T, alias_h<SCHEME, T>()> L;
     __shared__ int s_tile;
     DevState* st = a.st;
     // Every word this launch reads from the device state is loaded at once (the

@@ -191,10 +191,29 @@ constexpr bool faces_read_sound_speed()
 #ifndef DENSE_WAVES_CW
 #define DENSE_WAVES_CW 3
 #endif
-template <int SCHEME>
+// The Lax-Friedrichs upwind5 schemes on 512-thread tiles keep a face's h in
+// registers until every face of the tile has read the cell records, then write
+// the h arrays over those records (one more barrier).  A 32x15 tile then takes
+// 51 KB of LDS instead of 83 KB and, at 4 waves per SIMD (128 VGPRs), two
+// workgroups fit on a CU instead of one.  On 256-thread tiles the same change
+// measured 1-3 % slower (the 128-VGPR cap spills, and 3 workgroups of 46 KB
+// already hide the load phase); the other schemes need more than 128 VGPRs,
+// so their LDS never limits the workgroups per CU.
+#ifndef DENSE_ALIAS_H
+#define DENSE_ALIAS_H 1
+#endif
+template <int SCHEME, class T>
+constexpr bool alias_h()
+{
+    return DENSE_ALIAS_H && T::NT >= 512 && (SCHEME == S_LLF_UP5 || SCHEME == S_GLF_UP5);
+}
+
+template <int SCHEME, class T>
 constexpr int min_waves()
 {
-    return (SCHEME == S_CH || SCHEME == S_SC) ? DENSE_WAVES_CHAR : DENSE_WAVES_CW;
+    return (SCHEME == S_CH || SCHEME == S_SC) ? DENSE_WAVES_CHAR
+           : alias_h<SCHEME, T>()             ? 4
+                                              : DENSE_WAVES_CW;
 }
 
 // Per-XCD tile queue of a persistent launch: queue x owns the contiguous
@@ -206,11 +225,12 @@ __device__ __forceinline__ void queue_range(int x, int n, int& lo, int& hi)
     hi = lo + q + (x < r ? 1 : 0);
 }
 
-// LDS footprint of one tile.
-template <class T>
+// LDS footprint of one tile.  h at x faces [4][TY][TX+1], then y faces
+// [4][TY+1][TX]: in sh, or over the cell records when ALIAS.
+template <class T, bool ALIAS = false>
 struct TileLds {
-    double s[REC_N * T::NC];             // cell records of the tile + halo
-    double sh[4 * (T::NXF + T::NYF)];    // h at x faces [4][TY][TX+1], then y faces [4][TY+1][TX]
+    double s[REC_N * T::NC];                          // cell records of the tile + halo
+    double sh[ALIAS ? 1 : 4 * (T::NXF + T::NYF)];
 };
 
 // One tile of one RK stage.  `a` describes the block (pointers of this stage,
@@ -220,14 +240,16 @@ struct TileLds {
 // next tile index while the faces compute).  Ends with a barrier.
 template <int SCHEME, class T, class AfterLoad>
 __device__ __forceinline__ void run_tile(const StageArgs& a, int tile, double ai, double aj, double dt, double t,
-                                         TileLds<T>& L, unsigned long long& ei, unsigned long long& ej,
+                                         TileLds<T, alias_h<SCHEME, T>()>& L, unsigned long long& ei,
+                                         unsigned long long& ej,
                                          AfterLoad after_load)
 {
     constexpr int TX = T::TX, TY = T::TY, NT = T::NT, W = T::W, NC = T::NC;
     static_assert(TX * TY <= NT, "one update cell per thread");
     double* s = L.s;
-    double* shx = L.sh;
-    double* shy = L.sh + 4 * T::NXF;
+    constexpr bool ALIAS = alias_h<SCHEME, T>();
+    double* shx = ALIAS ? L.s : L.sh;
+    double* shy = shx + 4 * T::NXF;
     const int tid = threadIdx.x;
     const int ni = a.ni, nj = a.nj;
     const int x0 = (tile % a.tiles_x) * TX, y0 = (tile / a.tiles_x) * TY;
@@ -262,59 +284,70 @@ __device__ __forceinline__ void run_tile(const StageArgs& a, int tile, double ai
     }
 
     // ---- 2. faces: x faces (fx in [0,TX], ly in [0,TY)), then y faces
+    const auto put_x = [&](const double* h, int o) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) shx[k * T::NXF + o] = h[k];
+    };
+    const auto put_y = [&](const double* h, int o) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) shy[k * T::NYF + o] = h[k];
+    };
+    double hx[8], hy[8];
+    int ox = -1, oy = -1;  // this thread's x / y face in the h arrays, -1 if none
     if constexpr (T::DIRSPLIT) {
         if (tid < NT / 2) {
             const int fx = tid % (TX + 1), lyx = tid / (TX + 1);
-            double hx[8];
             if (tid < T::NXF && x0 + fx <= ni && y0 + lyx < nj) {
                 eval_face<SCHEME, 0, NC, 1>(s, (lyx + 3) * W + fx, a, ai, hx);
-#pragma unroll
-                for (int k = 0; k < 4; k++) shx[k * T::NXF + lyx * (TX + 1) + fx] = hx[k];
+                ox = lyx * (TX + 1) + fx;
+                if constexpr (!ALIAS) put_x(hx, ox);
             }
         } else {
             const int f = tid - NT / 2;
             const int lxy = f % TX, fy = f / TX;
-            double hy[8];
             if (f < T::NYF && x0 + lxy < ni && y0 + fy <= nj) {
                 eval_face<SCHEME, 1, NC, W>(s, fy * W + lxy + 3, a, aj, hy);
-#pragma unroll
-                for (int k = 0; k < 4; k++) shy[k * T::NYF + fy * TX + lxy] = hy[k];
+                oy = fy * TX + lxy;
+                if constexpr (!ALIAS) put_y(hy, oy);
             }
         }
     } else {
-    {
         const int fx = tid % (TX + 1), lyx = tid / (TX + 1);
-        double hx[8];
-        const bool vx = tid < T::NXF && x0 + fx <= ni && y0 + lyx < nj;
-        if (vx) {
+        if (tid < T::NXF && x0 + fx <= ni && y0 + lyx < nj) {
             eval_face<SCHEME, 0, NC, 1>(s, (lyx + 3) * W + fx, a, ai, hx);
-#pragma unroll
-            for (int k = 0; k < 4; k++) shx[k * T::NXF + lyx * (TX + 1) + fx] = hx[k];
+            ox = lyx * (TX + 1) + fx;
+            if constexpr (!ALIAS) put_x(hx, ox);
         }
-    }
-    {
         const int lxy = tid % TX, fy = tid / TX;
-        double hy[8];
-        const bool vy = tid < T::NYF && x0 + lxy < ni && y0 + fy <= nj;
-        if (vy) {
+        if (tid < T::NYF && x0 + lxy < ni && y0 + fy <= nj) {
             eval_face<SCHEME, 1, NC, W>(s, fy * W + lxy + 3, a, aj, hy);
-#pragma unroll
-            for (int k = 0; k < 4; k++) shy[k * T::NYF + fy * TX + lxy] = hy[k];
+            oy = fy * TX + lxy;
+            if constexpr (!ALIAS) put_y(hy, oy);
         }
     }
+    const int ic = (lyu + 3) * W + lxu + 3;  // the update cell's record: stage input U^(s-1)
+    double qin[4] = {0, 0, 0, 0};
+    if constexpr (ALIAS) {
+        // the h arrays overwrite the records: take the update cell's input first
+        if (vu) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) qin[k] = s[k * NC + ic];
+        }
+        __syncthreads();
+        if (ox >= 0) put_x(hx, ox);
+        if (oy >= 0) put_y(hy, oy);
     }
     __syncthreads();
 
     // ---- 3. RK stage update, time_advance_general.f90:35-54 / :113-135 / :195-217
     if (vu) {
-        const int ic = (lyu + 3) * W + lxu + 3;  // the cell's record: stage input U^(s-1)
         double q[4], qs[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const double dh_dxi =
                 shx[k * T::NXF + lyu * (TX + 1) + lxu + 1] - shx[k * T::NXF + lyu * (TX + 1) + lxu];
             const double dh_deta = shy[k * T::NYF + (lyu + 1) * TX + lxu] - shy[k * T::NYF + lyu * TX + lxu];
-            const double q1 = s[k * NC + ic];
+            const double q1 = ALIAS ? qin[k] : s[k * NC + ic];
             qs[k] = q1;
             if (a.stage == 1) {
                 q[k] = q1 - dt / a.dx * dh_dxi - dt / a.dy * dh_deta;

@@ -63,6 +63,25 @@ def test_batch_schedule_invariance(tile, order, grid):
         assert np.array_equal(a, c)
 
 
+@pytest.mark.parametrize("splitting", [101, 102])
+@pytest.mark.parametrize("tile", [0, 3])
+def test_batch_upwind5_equals_single_instances(splitting, tile):
+    """Lax-Friedrichs + upwind5 ensembles, including the 32x15 tiles whose face
+    fluxes overwrite the cell records in LDS, equal each instance alone."""
+    B = _batch_mod()
+    cfgs, states, insts = _ensemble(n=5, seed=5, splitting=splitting, recon=251)
+    steps = 6
+    print(f"batch {splitting}/251 tile {tile}: sizes {[(i.ni, i.nj) for i in insts]}", flush=True)
+    with B.Batch(cfgs) as b:
+        b.set_schedule(tile, 0, 100)
+        for i, c in enumerate(states):
+            b.set_state(i, c)
+        b.run(steps)
+        for i in range(len(cfgs)):
+            ref, _ = _alone(cfgs[i], states[i], steps)
+            assert np.array_equal(b.interior(i), ref), f"instance {i} ({insts[i].kind})"
+
+
 def test_batch_autotune_restores_state():
     B = _batch_mod()
     cfgs, states, _ = _ensemble(n=4, seed=3)
