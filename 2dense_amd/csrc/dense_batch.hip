@@ -24,7 +24,7 @@ template <int SCHEME, class T>
 __global__ void __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(min_waves<SCHEME, T>())))
 batch_stage_kernel(BatchArgs b)
 {
-    __shared__ TileLds<T, alias_h<SCHEME, T>()> L;
+    __shared__ TileLds<SCHEME, T> L;
     __shared__ StageArgs sa;  // the current instance, as the single-block kernel sees its block
     __shared__ double s_ai, s_aj, s_dt, s_t;
     __shared__ int s_tile, s_last;

@@ -14,7 +14,7 @@ template <int SCHEME, class T>
 __global__ void __launch_bounds__(T::NT) __attribute__((amdgpu_waves_per_eu(min_waves<SCHEME, T>())))
 stage_kernel(StageArgs a)
 {
-    __shared__ TileLds<T, alias_h<SCHEME, T>()> L;
+    __shared__ TileLds<SCHEME, T> L;
     __shared__ int s_tile;
     DevState* st = a.st;
     // Every word this launch reads from the device state is loaded at once (the

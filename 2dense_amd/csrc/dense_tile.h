@@ -225,13 +225,53 @@ __device__ __forceinline__ void queue_range(int x, int n, int& lo, int& hi)
     hi = lo + q + (x < r ? 1 : 0);
 }
 
-// LDS footprint of one tile.  h at x faces [4][TY][TX+1], then y faces
-// [4][TY+1][TX]: in sh, or over the cell records when ALIAS.
-template <class T, bool ALIAS = false>
+// Steger-Warming: a cell's split fluxes F+ and F- (sw_flux: 10 divisions and
+// 3 square roots) are computed once per sweep direction by the tile loader into
+// LDS bands, instead of once for each of the six faces whose stencil holds the
+// cell.  Bands: the x-split of the tile's TY rows (all W columns), the y-split
+// of its TX columns (all H rows), and the conserved state of the tile's own
+// cells for the update; the h arrays overwrite the bands after the faces.
+#ifndef DENSE_SW_BANDS
+#define DENSE_SW_BANDS 1
+#endif
+template <int SCHEME>
+constexpr bool split_in_loader()
+{
+    return DENSE_SW_BANDS && SCHEME >= S_SW_UP5 && SCHEME <= S_SW_Z;
+}
+
+// LDS footprint of one tile.  Cell records [REC_N][H][W], or with BANDS the
+// conserved state [4][TY][TX], the x band [8][TY][W] (F+ then F-) and the y band
+// [8][H][TX].  h at x faces [4][TY][TX+1], then y faces [4][TY+1][TX]: in sh,
+// over the cell records (ALIAS) or over the bands (BANDS).
+template <int SCHEME, class T>
 struct TileLds {
-    double s[REC_N * T::NC];                          // cell records of the tile + halo
-    double sh[ALIAS ? 1 : 4 * (T::NXF + T::NYF)];
+    static constexpr bool BANDS = split_in_loader<SCHEME>();
+    static constexpr bool ALIAS = alias_h<SCHEME, T>();
+    static constexpr int XB = T::W * T::TY, YB = T::TX * T::H;  // band plane sizes
+    static constexpr int NS = BANDS ? 4 * T::TX * T::TY + 8 * (XB + YB) : REC_N * T::NC;
+    static_assert(!BANDS || 4 * (T::NXF + T::NYF) <= 8 * (XB + YB), "h arrays alias the bands");
+    double s[NS];
+    double sh[(ALIAS || BANDS) ? 1 : 4 * (T::NXF + T::NYF)];
 };
+
+// One Steger-Warming face from the bands: the 6 split fluxes of its stencil
+// are band[base + m*st], m = 0..5 (cells f-3 .. f+2), as face_cw_sw forms them.
+template <int RECON>
+__device__ __forceinline__ void face_sw_bands(const double* __restrict__ band, int plane, int base, int st,
+                                              double* h)
+{
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double lfp[5], lfm[5];
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            lfp[i] = band[k * plane + base + i * st];
+            lfm[i] = band[(4 + k) * plane + base + (5 - i) * st];
+        }
+        h[k] = recon<RECON>(lfp) + recon<RECON>(lfm);
+    }
+}
 
 // One tile of one RK stage.  `a` describes the block (pointers of this stage,
 // grid, metrics, boundaries); ai/aj are the stage's global wave speeds, dt the
@@ -240,7 +280,7 @@ struct TileLds {
 // next tile index while the faces compute).  Ends with a barrier.
 template <int SCHEME, class T, class AfterLoad>
 __device__ __forceinline__ void run_tile(const StageArgs& a, int tile, double ai, double aj, double dt, double t,
-                                         TileLds<T, alias_h<SCHEME, T>()>& L, unsigned long long& ei,
+                                         TileLds<SCHEME, T>& L, unsigned long long& ei,
                                          unsigned long long& ej,
                                          AfterLoad after_load)
 {
@@ -248,7 +288,13 @@ __device__ __forceinline__ void run_tile(const StageArgs& a, int tile, double ai
     static_assert(TX * TY <= NT, "one update cell per thread");
     double* s = L.s;
     constexpr bool ALIAS = alias_h<SCHEME, T>();
-    double* shx = ALIAS ? L.s : L.sh;
+    constexpr bool BANDS = split_in_loader<SCHEME>();
+    constexpr bool HOLD = ALIAS || BANDS;  // h waits in registers, then overwrites the records / bands
+    constexpr int XB = TileLds<SCHEME, T>::XB, YB = TileLds<SCHEME, T>::YB;
+    double* sc = L.s;                    // BANDS: conserved state of the tile's cells
+    double* xs = L.s + 4 * TX * TY;      // BANDS: x band
+    double* ys = xs + 8 * XB;            // BANDS: y band
+    double* shx = BANDS ? xs : ALIAS ? L.s : L.sh;
     double* shy = shx + 4 * T::NXF;
     const int tid = threadIdx.x;
     const int ni = a.ni, nj = a.nj;
@@ -260,13 +306,42 @@ __device__ __forceinline__ void run_tile(const StageArgs& a, int tile, double ai
         const int gx = x0 + lx - 3, gy = y0 + ly - 3;
         const bool inx = (unsigned)gx < (unsigned)ni, iny = (unsigned)gy < (unsigned)nj;
         if ((!inx && !iny) || gx >= ni + 3 || gy >= nj + 3) continue;
+        if constexpr (BANDS) {
+            if ((unsigned)(ly - 3) >= (unsigned)TY && (unsigned)(lx - 3) >= (unsigned)TX) continue;
+        }
         double c[4], pr[4];
         if (inx && iny) load_interior<faces_read_sound_speed<SCHEME>()>(a, a.us, gx, gy, c, pr);
         else ghost_cell(a, a.us, gx, gy, t, c, pr);
+        if constexpr (BANDS) {
+            const bool bx = (unsigned)(ly - 3) < (unsigned)TY, by = (unsigned)(lx - 3) < (unsigned)TX;
+            if (bx && by) {
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            s[k * NC + idx] = c[k];
-            s[(4 + k) * NC + idx] = pr[k];
+                for (int k = 0; k < 4; k++) sc[k * TX * TY + (ly - 3) * TX + lx - 3] = c[k];
+            }
+            if (bx) {
+                double fp[4], fm[4];
+                sw_flux<0>(a.gamma, c[0], pr[0], pr[1], pr[3], fp, fm);
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    xs[k * XB + (ly - 3) * W + lx] = fp[k];
+                    xs[(4 + k) * XB + (ly - 3) * W + lx] = fm[k];
+                }
+            }
+            if (by) {
+                double fp[4], fm[4];
+                sw_flux<1>(a.gamma, c[0], pr[0], pr[1], pr[3], fp, fm);
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    ys[k * YB + ly * TX + lx - 3] = fp[k];
+                    ys[(4 + k) * YB + ly * TX + lx - 3] = fm[k];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                s[k * NC + idx] = c[k];
+                s[(4 + k) * NC + idx] = pr[k];
+            }
         }
     }
     __syncthreads();
@@ -298,40 +373,46 @@ __device__ __forceinline__ void run_tile(const StageArgs& a, int tile, double ai
         if (tid < NT / 2) {
             const int fx = tid % (TX + 1), lyx = tid / (TX + 1);
             if (tid < T::NXF && x0 + fx <= ni && y0 + lyx < nj) {
-                eval_face<SCHEME, 0, NC, 1>(s, (lyx + 3) * W + fx, a, ai, hx);
+                if constexpr (BANDS) face_sw_bands<SCHEME - S_SW_UP5>(xs, XB, lyx * W + fx, 1, hx);
+                else eval_face<SCHEME, 0, NC, 1>(s, (lyx + 3) * W + fx, a, ai, hx);
                 ox = lyx * (TX + 1) + fx;
-                if constexpr (!ALIAS) put_x(hx, ox);
+                if constexpr (!HOLD) put_x(hx, ox);
             }
         } else {
             const int f = tid - NT / 2;
             const int lxy = f % TX, fy = f / TX;
             if (f < T::NYF && x0 + lxy < ni && y0 + fy <= nj) {
-                eval_face<SCHEME, 1, NC, W>(s, fy * W + lxy + 3, a, aj, hy);
+                if constexpr (BANDS) face_sw_bands<SCHEME - S_SW_UP5>(ys, YB, fy * TX + lxy, TX, hy);
+                else eval_face<SCHEME, 1, NC, W>(s, fy * W + lxy + 3, a, aj, hy);
                 oy = fy * TX + lxy;
-                if constexpr (!ALIAS) put_y(hy, oy);
+                if constexpr (!HOLD) put_y(hy, oy);
             }
         }
     } else {
         const int fx = tid % (TX + 1), lyx = tid / (TX + 1);
         if (tid < T::NXF && x0 + fx <= ni && y0 + lyx < nj) {
-            eval_face<SCHEME, 0, NC, 1>(s, (lyx + 3) * W + fx, a, ai, hx);
+            if constexpr (BANDS) face_sw_bands<SCHEME - S_SW_UP5>(xs, XB, lyx * W + fx, 1, hx);
+            else eval_face<SCHEME, 0, NC, 1>(s, (lyx + 3) * W + fx, a, ai, hx);
             ox = lyx * (TX + 1) + fx;
-            if constexpr (!ALIAS) put_x(hx, ox);
+            if constexpr (!HOLD) put_x(hx, ox);
         }
         const int lxy = tid % TX, fy = tid / TX;
         if (tid < T::NYF && x0 + lxy < ni && y0 + fy <= nj) {
-            eval_face<SCHEME, 1, NC, W>(s, fy * W + lxy + 3, a, aj, hy);
+            if constexpr (BANDS) face_sw_bands<SCHEME - S_SW_UP5>(ys, YB, fy * TX + lxy, TX, hy);
+            else eval_face<SCHEME, 1, NC, W>(s, fy * W + lxy + 3, a, aj, hy);
             oy = fy * TX + lxy;
-            if constexpr (!ALIAS) put_y(hy, oy);
+            if constexpr (!HOLD) put_y(hy, oy);
         }
     }
     const int ic = (lyu + 3) * W + lxu + 3;  // the update cell's record: stage input U^(s-1)
     double qin[4] = {0, 0, 0, 0};
-    if constexpr (ALIAS) {
-        // the h arrays overwrite the records: take the update cell's input first
-        if (vu) {
+    if constexpr (HOLD) {
+        if constexpr (ALIAS) {
+            // the h arrays overwrite the records: take the update cell's input first
+            if (vu) {
 #pragma unroll
-            for (int k = 0; k < 4; k++) qin[k] = s[k * NC + ic];
+                for (int k = 0; k < 4; k++) qin[k] = s[k * NC + ic];
+            }
         }
         __syncthreads();
         if (ox >= 0) put_x(hx, ox);
@@ -347,7 +428,7 @@ __device__ __forceinline__ void run_tile(const StageArgs& a, int tile, double ai
             const double dh_dxi =
                 shx[k * T::NXF + lyu * (TX + 1) + lxu + 1] - shx[k * T::NXF + lyu * (TX + 1) + lxu];
             const double dh_deta = shy[k * T::NYF + (lyu + 1) * TX + lxu] - shy[k * T::NYF + lyu * TX + lxu];
-            const double q1 = ALIAS ? qin[k] : s[k * NC + ic];
+            const double q1 = BANDS ? sc[k * TX * TY + lyu * TX + lxu] : ALIAS ? qin[k] : s[k * NC + ic];
             qs[k] = q1;
             if (a.stage == 1) {
                 q[k] = q1 - dt / a.dx * dh_dxi - dt / a.dy * dh_deta;

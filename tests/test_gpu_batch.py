@@ -28,7 +28,7 @@ def _alone(cfg, cons, steps):
         return s.interior(), np.array(dts)
 
 
-@pytest.mark.parametrize("splitting,recon", [(101, 255), (104, 255), (105, 255)])
+@pytest.mark.parametrize("splitting,recon", [(101, 255), (103, 253), (104, 255), (105, 255)])
 def test_batch_equals_single_instances(splitting, recon):
     B = _batch_mod()
     cfgs, states, insts = _ensemble(splitting=splitting, recon=recon)

@@ -47,11 +47,13 @@ def test_gpu_matches_reference_golden(name, oracle):
 
 
 @pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
-@pytest.mark.parametrize("name", ["dmr_104", "ivc_101_253", "rt_105", "sv_101_255", "ivc_101_251", "ivc_102_251"])
+@pytest.mark.parametrize("name", ["dmr_104", "ivc_101_253", "rt_105", "sv_101_255", "ivc_101_251", "ivc_102_251",
+                                  "dmr_103_251", "ivc_103_255"])
 def test_gpu_tile_shape_invariance(name, tile, oracle):
     """Every tile shape gives the same bits (faces are independent; the only
     cross-tile reduction is a max).  The upwind5 cases cover the 512-thread
-    tiles whose face fluxes overwrite the cell records in LDS."""
+    tiles whose face fluxes overwrite the cell records in LDS; the
+    Steger-Warming cases the split-flux bands of every shape."""
     c = CASES[name]
     g = golden(name)
     o = _oracle_ic(oracle, c)
