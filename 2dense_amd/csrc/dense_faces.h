@@ -12,10 +12,7 @@
 namespace dense {
 
 // LDS record slots of one cell: 0..3 conserved (rho, rho u, rho v, E), then u, v, p, c.
-// The characteristic scheme's records (ROE_PRE) hold the cell's enthalpy for the
-// Roe average in the sound-speed slot, which its faces never read, and sqrt(rho)
-// in a ninth slot.
-enum { REC_U = 4, REC_V = 5, REC_P = 6, REC_C = 7, REC_N = 8, REC_H = 7, REC_SQ = 8 };
+enum { REC_U = 4, REC_V = 5, REC_P = 6, REC_C = 7, REC_N = 8 };
 
 // Component-wise reconstruction with Lax-Friedrichs splitting.
 //   LLF: alpha = |u_n|+c per cell (flux_splitting_LLF.f90:37);
@@ -82,12 +79,6 @@ __device__ __forceinline__ void face_cw_sw(const double* __restrict__ s, int bas
     }
 }
 
-// Total enthalpy of a cell for the Roe average, scheme_weno5th_CH.f90:44-62.
-__device__ __forceinline__ double roe_enthalpy(double gamma, double rho, double u, double v, double p)
-{
-    return p / (gamma - 1) / rho + 0.5 * (u * u + v * v) + p / rho;
-}
-
 // Characteristic WENO-Z (switch 104) and AdaWENO (switch 105) at one face.
 //   Roe average of cells f-1, f      scheme_weno5th_CH.f90:44-62
 //   eigenvectors L, R                scheme_weno5th_SC.f90:277-328 (x), :607-658 (y)
@@ -97,7 +88,7 @@ __device__ __forceinline__ double roe_enthalpy(double gamma, double rho, double 
 // The eigenvectors are written in the frame (normal, tangential); the
 // projection sums run over the conserved components in their original order
 // (rho, rho u, rho v, E) so each sum rounds as the reference's does.
-template <int DIR, bool ADAPT, int NC, int ST, bool ROE_PRE = false>
+template <int DIR, bool ADAPT, int NC, int ST>
 __device__ __forceinline__ void face_char(const double* __restrict__ s, int base, double alpha, double gamma,
                                           double delta, double* h)
 {
@@ -132,19 +123,11 @@ __device__ __forceinline__ void face_char(const double* __restrict__ s, int base
     double ut0 = 0, un0 = 0, a0 = 0, h0 = 0, uu0 = 0;
     if (chp || chm) {
         const int oL = base + 2 * ST, oR = base + 3 * ST;
-        const double ul = s[REC_U * NC + oL], vl = s[REC_V * NC + oL];
-        const double ur = s[REC_U * NC + oR], vr = s[REC_V * NC + oR];
-        double hl, hr, sl, sr;
-        if constexpr (ROE_PRE) {
-            hl = s[REC_H * NC + oL], hr = s[REC_H * NC + oR];
-            sl = s[REC_SQ * NC + oL], sr = s[REC_SQ * NC + oR];
-        } else {
-            const double rhol = s[oL], pl = s[REC_P * NC + oL];
-            const double rhor = s[oR], pr = s[REC_P * NC + oR];
-            hl = roe_enthalpy(gamma, rhol, ul, vl, pl);
-            hr = roe_enthalpy(gamma, rhor, ur, vr, pr);
-            sr = sqrt(rhor), sl = sqrt(rhol);
-        }
+        const double rhol = s[oL], ul = s[REC_U * NC + oL], vl = s[REC_V * NC + oL], pl = s[REC_P * NC + oL];
+        const double rhor = s[oR], ur = s[REC_U * NC + oR], vr = s[REC_V * NC + oR], pr = s[REC_P * NC + oR];
+        const double hl = pl / (gamma - 1) / rhol + 0.5 * (ul * ul + vl * vl) + pl / rhol;
+        const double hr = pr / (gamma - 1) / rhor + 0.5 * (ur * ur + vr * vr) + pr / rhor;
+        const double sr = sqrt(rhor), sl = sqrt(rhol);
         const double u0 = (sr * ur + sl * ul) / (sr + sl);
         const double v0 = (sr * vr + sl * vl) / (sr + sl);
         h0 = (sr * hr + sl * hl) / (sr + sl);

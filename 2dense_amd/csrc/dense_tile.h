@@ -165,24 +165,12 @@ __device__ __forceinline__ double cfl_dt(const StageArgs& a, double ai, double a
     return a.cfl * dt_xi * dt_eta / (dt_xi + dt_eta);
 }
 
-// Characteristic WENO-Z: the tile loader stores each cell's enthalpy and
-// sqrt(rho) for the Roe average (same expressions), which the four faces
-// around the cell would otherwise each recompute (3 divisions, 1 square root).
-#ifndef DENSE_ROE_PRE
-#define DENSE_ROE_PRE 1
-#endif
-template <int SCHEME>
-constexpr bool roe_pre()
-{
-    return DENSE_ROE_PRE && (SCHEME == S_CH || (DENSE_ROE_PRE > 1 && SCHEME == S_SC));
-}
-
 template <int SCHEME, int DIR, int NC, int ST>
 __device__ __forceinline__ void eval_face(const double* __restrict__ s, int base, const StageArgs& a, double alpha,
                                           double* h)
 {
-    if constexpr (SCHEME == S_CH) face_char<DIR, false, NC, ST, roe_pre<S_CH>()>(s, base, alpha, a.gamma, a.delta, h);
-    else if constexpr (SCHEME == S_SC) face_char<DIR, true, NC, ST, roe_pre<S_SC>()>(s, base, alpha, a.gamma, a.delta, h);
+    if constexpr (SCHEME == S_CH) face_char<DIR, false, NC, ST>(s, base, alpha, a.gamma, a.delta, h);
+    else if constexpr (SCHEME == S_SC) face_char<DIR, true, NC, ST>(s, base, alpha, a.gamma, a.delta, h);
     else if constexpr (SCHEME >= S_SW_UP5) face_cw_sw<DIR, SCHEME - S_SW_UP5, NC, ST>(s, base, a.gamma, h);
     else if constexpr (SCHEME >= S_GLF_UP5) face_cw_lf<DIR, false, SCHEME - S_GLF_UP5, NC, ST>(s, base, alpha, h);
     else face_cw_lf<DIR, true, SCHEME - S_LLF_UP5, NC, ST>(s, base, alpha, h);
@@ -261,7 +249,7 @@ struct TileLds {
     static constexpr bool BANDS = split_in_loader<SCHEME>();
     static constexpr bool ALIAS = alias_h<SCHEME, T>();
     static constexpr int XB = T::W * T::TY, YB = T::TX * T::H;  // band plane sizes
-    static constexpr int NS = BANDS ? 4 * T::TX * T::TY + 8 * (XB + YB) : (roe_pre<SCHEME>() ? REC_N + 1 : REC_N) * T::NC;
+    static constexpr int NS = BANDS ? 4 * T::TX * T::TY + 8 * (XB + YB) : REC_N * T::NC;
     static_assert(!BANDS || 4 * (T::NXF + T::NYF) <= 8 * (XB + YB), "h arrays alias the bands");
     double s[NS];
     double sh[(ALIAS || BANDS) ? 1 : 4 * (T::NXF + T::NYF)];
@@ -347,17 +335,6 @@ __device__ __forceinline__ void run_tile(const StageArgs& a, int tile, double ai
                     ys[k * YB + ly * TX + lx - 3] = fp[k];
                     ys[(4 + k) * YB + ly * TX + lx - 3] = fm[k];
                 }
-            }
-        } else if constexpr (roe_pre<SCHEME>()) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) s[k * NC + idx] = c[k];
-#pragma unroll
-            for (int k = 0; k < 3; k++) s[(4 + k) * NC + idx] = pr[k];
-            // cells f-1, f of an x face (columns 2..TX+3 of the tile rows) or a y face
-            if (((unsigned)(lx - 2) <= (unsigned)(TX + 1) && (unsigned)(ly - 3) < (unsigned)TY) ||
-                ((unsigned)(ly - 2) <= (unsigned)(TY + 1) && (unsigned)(lx - 3) < (unsigned)TX)) {
-                s[REC_H * NC + idx] = roe_enthalpy(a.gamma, c[0], pr[0], pr[1], pr[2]);
-                s[REC_SQ * NC + idx] = sqrt(c[0]);
             }
         } else {
 #pragma unroll
