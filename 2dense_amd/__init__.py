@@ -88,8 +88,11 @@ def load_library(path: str | None = None) -> C.CDLL:
     lib.dense_destroy.argtypes = [H]
     lib.dense_set_state.argtypes = [H, dp]
     lib.dense_get_state.argtypes = [H, dp, dp]
+    lib.dense_get_dump.argtypes = [H, dp]
     lib.dense_step.argtypes = [H, C.c_double, dp]
     lib.dense_run.argtypes = [H, C.c_int, C.c_double, C.c_int, dp, C.POINTER(C.c_int), dp, C.POINTER(C.c_int)]
+    lib.dense_run_saving.argtypes = [H, C.c_int, C.c_double, C.c_int, C.c_double, dp, dp, C.POINTER(C.c_int), dp,
+                                     C.POINTER(C.c_int)]
     lib.dense_max_eig.argtypes = [H, dp]
     lib.dense_last_run_gpu_ms.argtypes = [H, dp, C.POINTER(C.c_int)]
     lib.dense_update_params.argtypes = [H, P]
@@ -99,8 +102,8 @@ def load_library(path: str | None = None) -> C.CDLL:
     lib.dense_last_error.argtypes = [H]
     lib.dense_last_error.restype = C.c_char_p
     for name in ("dense_problem_defaults", "dense_create", "dense_destroy", "dense_set_state",
-                 "dense_get_state", "dense_step", "dense_run", "dense_max_eig", "dense_last_run_gpu_ms", "dense_update_params",
-                 "dense_autotune", "dense_get_theta", "dense_sync"):
+                 "dense_get_state", "dense_get_dump", "dense_step", "dense_run", "dense_run_saving", "dense_max_eig",
+                 "dense_last_run_gpu_ms", "dense_update_params", "dense_autotune", "dense_get_theta", "dense_sync"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -211,6 +214,13 @@ class Solver:
         _check(self.lib, self.h, self.lib.dense_get_state(self.h, _dp(cons), _dp(prim) if with_prim else None))
         return (cons, prim) if with_prim else cons
 
+    def get_dump(self) -> np.ndarray:
+        """cons(ni+8, nj+8, 4) as the reference's array holds it at a .dat2d dump:
+        the current interior, ghost layers of the last stage's input at t^n."""
+        cons = np.empty(self.padded_shape)
+        _check(self.lib, self.h, self.lib.dense_get_dump(self.h, _dp(cons)))
+        return cons
+
     def interior(self) -> np.ndarray:
         return self.get_state()[:, BF:-BF, BF:-BF].copy()
 
@@ -232,13 +242,29 @@ class Solver:
         t = C.c_double(self.current_time)
         st = C.c_int(self.current_step)
         taken = C.c_int(0)
-        hist = np.zeros(max(1, min(max_steps, 4096))) if want_dts else None
+        hist = np.zeros(max(1, max_steps)) if want_dts else None
         _check(self.lib, self.h, self.lib.dense_run(self.h, max_steps, total_time, total_step, C.byref(t),
                                                     C.byref(st), _dp(hist) if want_dts else None, C.byref(taken)))
         self.current_time, self.current_step = t.value, st.value
         if want_dts:
-            return taken.value, hist[: min(taken.value, hist.size)].copy()
+            return taken.value, hist[: taken.value].copy()
         return taken.value
+
+    def run_saving(self, max_steps: int, dt_save: float, elasp: float, total_time: float = float("inf"),
+                   total_step: int = 2 ** 31 - 1):
+        """run() that also stops after the first step at which the save clock
+        elasp (advanced by dt each step) reaches dt_save.  Returns (steps taken,
+        their dts, the advanced save clock)."""
+        t = C.c_double(self.current_time)
+        st = C.c_int(self.current_step)
+        el = C.c_double(elasp)
+        taken = C.c_int(0)
+        hist = np.zeros(max(1, max_steps))
+        _check(self.lib, self.h, self.lib.dense_run_saving(self.h, max_steps, total_time, total_step, dt_save,
+                                                           C.byref(el), C.byref(t), C.byref(st), _dp(hist),
+                                                           C.byref(taken)))
+        self.current_time, self.current_step = t.value, st.value
+        return taken.value, hist[: taken.value].copy(), el.value
 
     def last_run_gpu_ms(self):
         """(GPU milliseconds of the last run(), RK-stage launches that did work)."""

@@ -21,7 +21,7 @@ from . import (BC_EXTRAP, BC_PERIODIC, BC_REFLECT, BF, Cfg, DenseError, LLF, WEN
                make_cfg)
 
 
-IC_VORTEX, IC_SHOCK_VORTEX = 1, 2
+IC_VORTEX, IC_SHOCK_VORTEX, IC_SHOCK_VORTEX_REF = 1, 2, 3
 
 
 class IcDesc(C.Structure):
@@ -218,6 +218,8 @@ def ic_desc(inst) -> IcDesc:
     elif inst.kind == "shock_vortex":
         d.kind, d.mach, d.xc, d.yc = IC_SHOCK_VORTEX, P["mach"], P["xc"], P["yc"]
         d.eps, d.rc, d.alpha, d.xs = P.get("eps", 0.3), P.get("rc", 0.05), P.get("alpha", 0.204), P.get("xs", 0.5)
+    elif inst.kind == "shock_vortex_ref":  # the reference's benchmark 9, no free parameters
+        d.kind = IC_SHOCK_VORTEX_REF
     else:
         raise ValueError(f"unknown instance kind {inst.kind!r}")
     return d
@@ -225,7 +227,7 @@ def ic_desc(inst) -> IcDesc:
 
 @dataclass
 class Instance:
-    kind: str          # "vortex" or "shock_vortex"
+    kind: str          # "vortex", "shock_vortex" or "shock_vortex_ref"
     ni: int
     nj: int
     params: dict

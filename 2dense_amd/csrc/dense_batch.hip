@@ -100,6 +100,7 @@ batch_stage_kernel(BatchArgs b)
     }
     __syncthreads();
     if (!s_last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the other blocks' maxima (see launch_epilogue)
     const int out = (seq + 1) & 3, nxt = (seq + 2) & 3;
     const unsigned long long inf_bits = 0x7ff0000000000000ull;
     for (int i = tid; i < b.ninst; i += T::NT) {
@@ -595,6 +596,36 @@ __device__ __forceinline__ void ic_shock_vortex(const dense_ic& P, double gamma,
     }
 }
 
+// The reference's own shock/vortex initial state, init_shockvortex_general.f90:34-56,
+// operation for operation: its single-precision literals widened as Fortran
+// widens them (centre 0.25, 0.5; eps 0.3, r_c 0.05, alpha 0.204; Mach 1.1; the
+// cell centre (i - 0.5) formed in single precision, :21-22) and the printed
+// post-shock constants (:47-50) instead of the Rankine-Hugoniot relations.
+// (i, j) are 0-based interior indices.
+__device__ __forceinline__ void ic_shock_vortex_ref(double gamma, int i, int j, double dx, double dy, double* q)
+{
+    const double x_c = (double)0.25f, y_c = (double)0.5f;
+    const double epsl = (double)0.3f, r_c = (double)0.05f, alpha = (double)0.204f;
+    const double x = (double)((float)(i + 1) - 0.5f) * dx;
+    const double y = (double)((float)(j + 1) - 0.5f) * dy;
+    const double r = sqrt((x - x_c) * (x - x_c) + (y - y_c) * (y - y_c));
+    const double tau = r / r_c;
+    double rho, u, v, p;
+    if (x < (double)0.5f) {
+        rho = pow(1.0 - (gamma - 1.0) * (epsl * epsl) * exp(2.0 * alpha * (1.0 - tau * tau)) / (4.0 * alpha * gamma),
+                  1.0 / (gamma - 1.0));
+        u = (double)1.1f * sqrt(gamma) + epsl * tau * exp(alpha * (1.0 - tau * tau)) * (y - y_c) / r;
+        v = -(epsl * tau * exp(alpha * (1.0 - tau * tau)) * (x - x_c) / r);
+        p = pow(rho, gamma);
+    } else {
+        rho = 1.169082;
+        u = 1.113299;
+        v = 0.0;
+        p = 1.245000;
+    }
+    cons_of(rho, u, v, p, gamma, q);
+}
+
 // blockIdx.y = instance; the x blocks stride over its whole plane (row pitch
 // included), so padding columns are written as zeros like dense_batch_set_state.
 __global__ void __launch_bounds__(256) batch_ic_kernel(const BatchInst* __restrict__ inst,
@@ -611,8 +642,10 @@ __global__ void __launch_bounds__(256) batch_ic_kernel(const BatchInst* __restri
             const double x = ((double)i + 0.5) * I.dx, y = ((double)j + 0.5) * I.dy;
             if (P.kind == DENSE_IC_VORTEX)
                 ic_vortex(P, I.gamma, x, y, q);
-            else
+            else if (P.kind == DENSE_IC_SHOCK_VORTEX)
                 ic_shock_vortex(P, I.gamma, x, y, q);
+            else
+                ic_shock_vortex_ref(I.gamma, i, j, I.dx, I.dy, q);
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) U[k * n + c] = q[k];
@@ -633,7 +666,7 @@ extern "C" int dense_batch_init_ic(dense_batch_handle b, const dense_ic* ics)
         } else if (P.kind == DENSE_IC_SHOCK_VORTEX) {
             if (!(P.mach >= 1.0) || !(P.rc > 0.0))
                 return bfail(b, DENSE_ERR_ARG, "instance %d: shock/vortex needs mach >= 1 and rc > 0", i);
-        } else {
+        } else if (P.kind != DENSE_IC_SHOCK_VORTEX_REF) {
             return bfail(b, DENSE_ERR_ARG, "instance %d: unknown initial-state kind %d", i, P.kind);
         }
         maxplane = std::max(maxplane, b->inst[i].plane);

@@ -27,10 +27,13 @@ struct dense_solver {
     double* pad_dev = nullptr;    // padded cons + prim for get_state
     double* theta_dev = nullptr;  // AdaWENO indicator at the faces, for get_theta
     bool stepped = false;         // a step ran since the state or the parameters were set
+    bool stage_in_valid = false;  // U[stages-1] holds the last step's last-stage input
     double* pad_host = nullptr;   // pinned staging
     double* stage_host = nullptr; // pinned interior staging for set_state
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph = nullptr;      // GRAPH_CHUNK steps
     int graph_steps = 0;
+    hipGraphExec_t graph_rem = nullptr;  // the remainder max_steps % GRAPH_CHUNK of a run
+    int graph_rem_steps = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;  // bracket the last dense_run on the stream
     double last_run_ms = 0.0;
     int last_run_launches = 0;
@@ -228,6 +231,9 @@ static void release_graph(dense_solver* h)
     if (h->graph) (void)hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
     h->graph_steps = 0;
+    if (h->graph_rem) (void)hipGraphExecDestroy(h->graph_rem);
+    h->graph_rem = nullptr;
+    h->graph_rem_steps = 0;
 }
 
 extern "C" int dense_create(const dense_cfg* cfg, dense_handle* out)
@@ -348,6 +354,7 @@ extern "C" int dense_set_state(dense_handle h, const double* cons)
         }
     HIPCHK(h, hipMemcpyAsync(h->U[0], h->stage_host, sizeof(double) * 4 * h->plane, hipMemcpyHostToDevice, h->stream));
     h->stepped = false;
+    h->stage_in_valid = false;
     // fresh reduction ring, then the wave speeds of the uploaded state
     int rc = read_state(h);
     if (rc) return rc;
@@ -365,9 +372,10 @@ extern "C" int dense_set_state(dense_handle h, const double* cons)
     return DENSE_OK;
 }
 
-extern "C" int dense_get_state(dense_handle h, double* cons, double* prim)
+// Padded cons (and prim, entropy included) into the caller's arrays: interior
+// from U^n, ghost layers from the boundary rules applied to `gsrc` at time tg.
+static int padded_out(dense_solver* h, const double* gsrc, double tg, double* cons, double* prim)
 {
-    if (!h || !cons) return fail(h, DENSE_ERR_ARG, "null argument");
     const int ni = h->cfg.ni, nj = h->cfg.nj;
     const size_t np = (size_t)(ni + 2 * BF) * (nj + 2 * BF);
     if (!h->pad_dev && hipMalloc(&h->pad_dev, sizeof(double) * 9 * np) != hipSuccess)
@@ -375,15 +383,13 @@ extern "C" int dense_get_state(dense_handle h, double* cons, double* prim)
     if (!h->pad_host && hipHostMalloc(&h->pad_host, sizeof(double) * 9 * np) != hipSuccess)
         return fail(h, DENSE_ERR_NOMEM, "pinned padded buffer allocation failed");
     if (h->bc_rows) {
-        // inflow ghost cells at the solver's clock
-        int rc = read_state(h);
-        if (rc) return rc;
-        ssl_clock_terms(h->st_host->t, h->bct_host);
+        // inflow ghost cells at tg
+        ssl_clock_terms(tg, h->bct_host);
         HIPCHK(h, hipMemcpyAsync(&h->st->bc_t[0], h->bct_host, sizeof(double) * 2, hipMemcpyHostToDevice, h->stream));
     }
     StageArgs a = base_args(h);
     a.us = h->U[0];
-    HIPCHK(h, launch_fill_padded(a, h->pad_dev, h->pad_dev + 4 * np, h->stream));
+    HIPCHK(h, launch_fill_padded(a, gsrc, tg, h->pad_dev, h->pad_dev + 4 * np, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->pad_host, h->pad_dev, sizeof(double) * 9 * np, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     std::memcpy(cons, h->pad_host, sizeof(double) * 4 * np);
@@ -402,6 +408,29 @@ extern "C" int dense_get_state(dense_handle h, double* cons, double* prim)
     return DENSE_OK;
 }
 
+extern "C" int dense_get_state(dense_handle h, double* cons, double* prim)
+{
+    if (!h || !cons) return fail(h, DENSE_ERR_ARG, "null argument");
+    int rc = read_state(h);
+    if (rc) return rc;
+    return padded_out(h, h->U[0], h->st_host->t, cons, prim);
+}
+
+// cons as the reference's array holds it at a save: interior = the current
+// state; ghost layers = the last boundary pass, i.e. the boundary rules on the
+// last stage's input at t^n (general_main.f90:26-31, written by
+// write_date_binary at :43-44).
+extern "C" int dense_get_dump(dense_handle h, double* cons)
+{
+    if (!h || !cons) return fail(h, DENSE_ERR_ARG, "null argument");
+    int rc = read_state(h);
+    if (rc) return rc;
+    if (!h->stage_in_valid) return padded_out(h, h->U[0], h->st_host->t, cons, nullptr);
+    if (h->cfg.stages < 2)
+        return fail(h, DENSE_ERR_UNSUPPORTED, "one-stage steps keep no copy of the last stage's input");
+    return padded_out(h, h->U[h->cfg.stages - 1], h->st_host->t_prev, cons, nullptr);
+}
+
 extern "C" int dense_step(dense_handle h, double current_time, double* dt)
 {
     if (!h) return fail(h, DENSE_ERR_ARG, "null handle");
@@ -413,14 +442,18 @@ extern "C" int dense_step(dense_handle h, double current_time, double* dt)
     rc = read_state(h);
     if (rc) return rc;
     h->stepped = true;
+    h->stage_in_valid = true;
     if (dt) *dt = h->st_host->dt;
     if (h->st_host->error) return fail(h, DENSE_ERR_NONFINITE, "non-finite wave speed after step");
     return DENSE_OK;
 }
 
-static int build_graph(dense_solver* h, int nsteps)
+// Capture nsteps steps into *exec (replacing what it held).
+static int build_graph(dense_solver* h, int nsteps, hipGraphExec_t* exec, int* exec_steps)
 {
-    release_graph(h);
+    if (*exec) (void)hipGraphExecDestroy(*exec);
+    *exec = nullptr;
+    *exec_steps = 0;
     hipGraph_t g = nullptr;
     HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < nsteps; i++) {
@@ -432,31 +465,58 @@ static int build_graph(dense_solver* h, int nsteps)
         }
     }
     HIPCHK(h, hipStreamEndCapture(h->stream, &g));
-    hipError_t e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+    hipError_t e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return fail(h, DENSE_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-    h->graph_steps = nsteps;
+    if (e != hipSuccess) {
+        *exec = nullptr;
+        return fail(h, DENSE_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    *exec_steps = nsteps;
     return DENSE_OK;
 }
 
-extern "C" int dense_run(dense_handle h, int max_steps, double total_time, int total_step, double* t, int* step,
-                         double* dt_hist, int* steps_taken)
+// The device-resident dt history holds at least n entries (the captured
+// graphs carry its address, so they are rebuilt after it moves).
+static int ensure_hist(dense_solver* h, int n)
 {
-    if (!h || !t || !step || max_steps < 0) return fail(h, DENSE_ERR_ARG, "bad arguments");
+    if (n <= h->hist_cap) return DENSE_OK;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double* p = nullptr;
+    if (hipMalloc(&p, sizeof(double) * (size_t)n) != hipSuccess)
+        return fail(h, DENSE_ERR_NOMEM, "dt history of %d steps: allocation failed", n);
+    (void)hipFree(h->dt_hist);
+    h->dt_hist = p;
+    h->hist_cap = n;
+    HIPCHK(h, hipMemcpyAsync(&h->st->hist_cap, &h->hist_cap, sizeof(int), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    release_graph(h);
+    return DENSE_OK;
+}
+
+constexpr int GRAPH_CHUNK = 16;
+
+// The loop general_main.f90:23-40 on the device: up to max_steps steps while
+// t < total_time, step < total_step and !(elasp >= dt_save), the clock, step
+// count and save clock advanced by the last stage of every step.
+static int run_steps(dense_solver* h, int max_steps, double total_time, int total_step, double dt_save,
+                     double* elasp, double* t, int* step, double* dt_hist, int* steps_taken)
+{
     if (steps_taken) *steps_taken = 0;
     if (max_steps == 0) return DENSE_OK;
+    double el = elasp ? *elasp : 0.0;
     if (h->bc_rows) {
         // the inflow needs the host's cos of t^n: one synchronised step at a time,
         // the clock advanced as the device advances it (t + dt)
         double tt = *t;
         int st = *step, taken = 0;
         HIPCHK(h, hipEventRecord(h->ev0, h->stream));
-        while (taken < max_steps && tt < total_time && st < total_step) {
+        while (taken < max_steps && tt < total_time && st < total_step && !(taken > 0 && el >= dt_save)) {
             double dt = 0.0;
             int rc = dense_step(h, tt, &dt);
             if (rc) return rc;
             if (dt_hist) dt_hist[taken] = dt;
             tt = tt + dt;
+            el = el + dt;
             st += 1;
             taken += 1;
         }
@@ -467,6 +527,7 @@ extern "C" int dense_run(dense_handle h, int max_steps, double total_time, int t
         HIPCHK(h, hipStreamSynchronize(h->stream));
         *t = tt;
         *step = st;
+        if (elasp) *elasp = el;
         if (steps_taken) *steps_taken = taken;
         float ms = 0.f;
         HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
@@ -474,30 +535,39 @@ extern "C" int dense_run(dense_handle h, int max_steps, double total_time, int t
         h->last_run_launches = taken * h->cfg.stages;
         return DENSE_OK;
     }
-    const int chunk = 16;
-    if (h->graph_steps != chunk) {
-        int rc = build_graph(h, chunk);
+    if (dt_hist) {
+        int rc = ensure_hist(h, max_steps);
         if (rc) return rc;
     }
-    // the device stops itself once the clock passes total_time/total_step
+    // whole chunks, then one graph of the remainder: no replayed step is idle
+    // unless the clock or the save clock stops the run early
+    const int full = max_steps / GRAPH_CHUNK, rem = max_steps % GRAPH_CHUNK;
+    if (full > 0 && h->graph_steps != GRAPH_CHUNK) {
+        int rc = build_graph(h, GRAPH_CHUNK, &h->graph, &h->graph_steps);
+        if (rc) return rc;
+    }
+    if (rem > 0 && h->graph_rem_steps != rem) {
+        int rc = build_graph(h, rem, &h->graph_rem, &h->graph_rem_steps);
+        if (rc) return rc;
+    }
+    // the device stops itself once the clock passes total_time/total_step/dt_save
     const long long want = (long long)*step + max_steps;  // no int overflow near INT_MAX
     const int step_end = want < total_step ? (int)want : total_step;
-    HIPCHK(h, launch_set_clock(h->st, *t, total_time, *step, step_end, h->stream));
-    const int launches = (max_steps + chunk - 1) / chunk;
+    HIPCHK(h, launch_set_clock(h->st, *t, total_time, *step, step_end, h->stream, 0.0, 0.0, el, dt_save));
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
-    for (int i = 0; i < launches; i++) HIPCHK(h, hipGraphLaunch(h->graph, h->stream));
+    for (int i = 0; i < full; i++) HIPCHK(h, hipGraphLaunch(h->graph, h->stream));
+    if (rem > 0) HIPCHK(h, hipGraphLaunch(h->graph_rem, h->stream));
     HIPCHK(h, hipEventRecord(h->ev1, h->stream));
     int rc = read_state(h);
     if (rc) return rc;
     const DevState& s = *h->st_host;
     const int taken = s.step - *step;
-    if (dt_hist && taken > 0) {
-        const int n = taken < h->hist_cap ? taken : h->hist_cap;
-        HIPCHK(h, hipMemcpy(dt_hist, h->dt_hist, sizeof(double) * n, hipMemcpyDeviceToHost));
-    }
+    if (dt_hist && taken > 0)
+        HIPCHK(h, hipMemcpy(dt_hist, h->dt_hist, sizeof(double) * (size_t)taken, hipMemcpyDeviceToHost));
     *t = s.t;
     *step = s.step;
-    if (taken > 0) h->stepped = true;
+    if (elasp) *elasp = s.elasp;
+    if (taken > 0) h->stepped = h->stage_in_valid = true;
     if (steps_taken) *steps_taken = taken;
     float ms = 0.f;
     HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
@@ -505,6 +575,20 @@ extern "C" int dense_run(dense_handle h, int max_steps, double total_time, int t
     h->last_run_launches = taken * h->cfg.stages;
     if (s.error) return fail(h, DENSE_ERR_NONFINITE, "non-finite wave speed at step %d", s.step);
     return DENSE_OK;
+}
+
+extern "C" int dense_run(dense_handle h, int max_steps, double total_time, int total_step, double* t, int* step,
+                         double* dt_hist, int* steps_taken)
+{
+    if (!h || !t || !step || max_steps < 0) return fail(h, DENSE_ERR_ARG, "bad arguments");
+    return run_steps(h, max_steps, total_time, total_step, INFINITY, nullptr, t, step, dt_hist, steps_taken);
+}
+
+extern "C" int dense_run_saving(dense_handle h, int max_steps, double total_time, int total_step, double dt_save,
+                                double* elasp, double* t, int* step, double* dt_hist, int* steps_taken)
+{
+    if (!h || !t || !step || !elasp || max_steps < 0) return fail(h, DENSE_ERR_ARG, "bad arguments");
+    return run_steps(h, max_steps, total_time, total_step, dt_save, elasp, t, step, dt_hist, steps_taken);
 }
 
 extern "C" int dense_last_run_gpu_ms(dense_handle h, double* ms, int* stage_launches)
@@ -575,7 +659,6 @@ extern "C" int dense_autotune(dense_handle h, int trial_steps, int* tile, double
     int rc = read_state(h);
     if (rc) return rc;
     const DevState st0 = *h->st_host;
-    const bool stepped0 = h->stepped;
     const size_t bytes = sizeof(double) * 4 * h->plane;
     double* save = nullptr;
     HIPCHK(h, hipMalloc(&save, bytes));
@@ -612,7 +695,10 @@ extern "C" int dense_autotune(dense_handle h, int trial_steps, int* tile, double
     }
     use_tile(rc ? keep : bt);
     const int rr = restore();
-    h->stepped = stepped0;
+    // the trials overwrote the stage buffers: theta and the dump ghosts of the
+    // last step are gone until the next step
+    h->stepped = false;
+    h->stage_in_valid = false;
     (void)hipFree(save);
     if (rc) return rc;
     if (rr) return rr;
@@ -666,3 +752,4 @@ extern "C" int dense_sync(dense_handle h)
 }
 
 extern "C" const char* dense_last_error(dense_handle h) { return h ? h->err : g_err; }
+

@@ -51,6 +51,8 @@ struct alignas(128) DevState {
     int step_end;
     int seq;                         // kernel sequence number: reads alpha[seq&3], writes alpha[(seq+1)&3]
     int active;                      // 1 while the current step runs
+    double elasp;                    // the save clock: sum of dt since the last Tecplot save ...
+    double elasp_end;                // ... run while !(elasp >= elasp_end) (general_main.f90:37,40)
     LaunchCounter blocks_done;       // last-block detection
     LaunchCounter qnext[8];          // per-XCD tile queues of the persistent stage kernel
     int error;                       // 1: non-finite wave speed
@@ -124,9 +126,16 @@ hipError_t launch_batch_max(const BatchArgs& b, hipStream_t s);
 hipError_t launch_theta(const StageArgs& a, double ai, double aj, double t, double* theta, hipStream_t s);
 hipError_t launch_stage(int scheme, int tile, const StageArgs& a, hipStream_t s);
 hipError_t launch_max(const StageArgs& a, hipStream_t s);
+// Set the device clock (t, step), the stop condition and the save clock
+// (elasp, elasp_end; infinite: no save stop), clear the dt history count.
 hipError_t launch_set_clock(DevState* st, double t, double t_end, int step, int step_end, hipStream_t s,
-                            double bc_t0 = 0.0, double bc_t1 = 0.0);
-hipError_t launch_fill_padded(const StageArgs& a, double* cons_pad, double* prim_pad, hipStream_t s);
+                            double bc_t0 = 0.0, double bc_t1 = 0.0, double elasp = 0.0,
+                            double elasp_end = __builtin_inf());
+// Padded reference-layout cons(ni+8,nj+8,4) and prim(ni+8,nj+8,5): interior
+// cells from a.us, ghost layers from the boundary rules applied to ghost_src
+// at time t_ghost (corners zero).
+hipError_t launch_fill_padded(const StageArgs& a, const double* ghost_src, double t_ghost, double* cons_pad,
+                              double* prim_pad, hipStream_t s);
 int tile_count(int tile, int ni, int nj, int* tiles_x);
 
 // dense_cfg.tile -> index of the tile shape list (dense_tile.h: 0 16x15,

@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(256) theta_kernel(StageArgs a, double alpha, d
 }
 
 __global__ void set_clock_kernel(DevState* st, double t, double t_end, int step, int step_end, double bc_t0,
-                                 double bc_t1)
+                                 double bc_t1, double elasp, double elasp_end)
 {
     st->bc_t[0] = bc_t0;
     st->bc_t[1] = bc_t1;
@@ -122,13 +122,18 @@ __global__ void set_clock_kernel(DevState* st, double t, double t_end, int step,
     st->t_end = t_end;
     st->step = step;
     st->step_end = step_end;
+    st->elasp = elasp;
+    st->elasp_end = elasp_end;
     st->hist_n = 0;
+    // the save clock is tested after each step (general_main.f90:40): at least one step runs
     st->active = (t < t_end) && (step < step_end) && !st->error;
 }
 
-// Padded reference-layout arrays cons(ni+8,nj+8,4), prim(ni+8,nj+8,5) of the
-// current state with ghost layers from the boundary rules at time st->t.
-__global__ void __launch_bounds__(256) fill_padded_kernel(StageArgs a, double* cons, double* prim)
+// Padded reference-layout arrays cons(ni+8,nj+8,4), prim(ni+8,nj+8,5): the
+// interior from a.us, the ghost layers from the boundary rules applied to gsrc
+// at time tg (the corners, which no boundary routine writes, zero).
+__global__ void __launch_bounds__(256) fill_padded_kernel(StageArgs a, const double* gsrc, double tg, double* cons,
+                                                          double* prim)
 {
     const int nip = a.ni + 2 * BF, njp = a.nj + 2 * BF;
     const long long np = (long long)nip * njp;
@@ -143,7 +148,7 @@ __global__ void __launch_bounds__(256) fill_padded_kernel(StageArgs a, double* c
         load_interior(a, a.us, gx, gy, c, pr);
         rho = c[0];
     } else if (inx || iny) {
-        ghost_cell(a, a.us, gx, gy, a.st->t, c, pr);
+        ghost_cell(a, gsrc, gx, gy, tg, c, pr);
         rho = c[0];
     }
 #pragma unroll
@@ -245,16 +250,19 @@ hipError_t launch_max(const StageArgs& a, hipStream_t s)
 }
 
 hipError_t launch_set_clock(DevState* st, double t, double t_end, int step, int step_end, hipStream_t s,
-                            double bc_t0, double bc_t1)
+                            double bc_t0, double bc_t1, double elasp, double elasp_end)
 {
-    hipLaunchKernelGGL(set_clock_kernel, dim3(1), dim3(1), 0, s, st, t, t_end, step, step_end, bc_t0, bc_t1);
+    hipLaunchKernelGGL(set_clock_kernel, dim3(1), dim3(1), 0, s, st, t, t_end, step, step_end, bc_t0, bc_t1, elasp,
+                       elasp_end);
     return hipGetLastError();
 }
 
-hipError_t launch_fill_padded(const StageArgs& a, double* cons, double* prim, hipStream_t s)
+hipError_t launch_fill_padded(const StageArgs& a, const double* ghost_src, double t_ghost, double* cons, double* prim,
+                              hipStream_t s)
 {
     long long np = (long long)(a.ni + 2 * BF) * (a.nj + 2 * BF);
-    hipLaunchKernelGGL(fill_padded_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, a, cons, prim);
+    hipLaunchKernelGGL(fill_padded_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, a, ghost_src, t_ghost,
+                       cons, prim);
     return hipGetLastError();
 }
 

@@ -117,15 +117,16 @@ __device__ __forceinline__ void block_max_to_slot(unsigned long long ei, unsigne
 
 // The last workgroup of a launch advances the kernel sequence number, clears
 // the slot the next launch reduces into, checks the new wave speeds and, at
-// the end of a time step, advances the device clock (general_main.f90:35-37).
-// Only atomics cross workgroups here: each block waits until its max
-// atomics have been performed (wait_vm_writes) before its counter increment,
-// so the last block reads final maxima.  (A workgroup-scope fence does not
-// wait for global operations on this target, and an agent-scope release
-// fence would write the L2 back.)  Stores to the state
-// arrays and the DevState fields reach the next launch through the kernel
-// boundary, so no device-wide fence (an L2 write-back per block on a
-// multi-XCD part) is needed.
+// the end of a time step, advances the device clock and the save clock
+// (general_main.f90:35-37).
+// Only atomics cross workgroups here.  Producer side: each block waits until
+// its max atomics have been performed (wait_vm_writes: agent-scope atomics
+// complete in issue order once vmcnt reaches zero) before its relaxed counter
+// increment; this avoids an agent-scope release fence, which would write the
+// L2 back on every block of a multi-XCD part.  Consumer side: the block that
+// takes the last count issues an agent-scope acquire fence (once per launch)
+// before it reads the maxima.  Stores to the state arrays and the DevState
+// fields reach the next launch through the kernel boundary.
 __device__ void launch_epilogue(DevState* st, int seq, bool end_of_step, double t, double dt, double* dt_hist)
 {
     __syncthreads();
@@ -135,6 +136,7 @@ __device__ void launch_epilogue(DevState* st, int seq, bool end_of_step, double 
     const unsigned int prev =
         __hip_atomic_fetch_add(&st->blocks_done.v, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (prev != gridDim.x - 1) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     __hip_atomic_store(&st->blocks_done.v, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
     for (int x = 0; x < 8; x++) __hip_atomic_store(&st->qnext[x].v, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -152,7 +154,9 @@ __device__ void launch_epilogue(DevState* st, int seq, bool end_of_step, double 
         st->t_prev = t;
         st->t = t + dt;
         st->step += 1;
-        st->active = (st->t < st->t_end) && (st->step < st->step_end) && !st->error;
+        const double el = st->elasp + dt;  // elasp_time = elasp_time + dt, general_main.f90:37
+        st->elasp = el;
+        st->active = (st->t < st->t_end) && (st->step < st->step_end) && !(el >= st->elasp_end) && !st->error;
     }
 }
 

@@ -1,6 +1,7 @@
 ! iso_c_binding interface to lib2dense_amd.so (include/dense_capi.h).
 ! A Fortran caller replaces the reference's per-stage hot-path calls
-! (general_main.f90:26-31) with one dense_step() per time step.
+! (general_main.f90:26-31) with one dense_step() per time step, or replays
+! the steps between two saves with one dense_run_saving().
 module dense_capi
   use iso_c_binding
   implicit none
@@ -48,6 +49,24 @@ module dense_capi
       type(c_ptr), value :: h
       real(c_double), intent(out) :: cons(*)
       type(c_ptr), value :: prim
+    end function
+
+    integer(c_int) function dense_get_dump(h, cons) bind(C, name="dense_get_dump")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: cons(*)
+    end function
+
+    integer(c_int) function dense_run_saving(h, max_steps, total_time, total_step, dt_save, elasp, t, step, &
+                                             dt_hist, steps_taken) bind(C, name="dense_run_saving")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      integer(c_int), value :: max_steps, total_step
+      real(c_double), value :: total_time, dt_save
+      real(c_double), intent(inout) :: elasp, t
+      integer(c_int), intent(inout) :: step
+      real(c_double), intent(out) :: dt_hist(*)
+      integer(c_int), intent(out) :: steps_taken
     end function
 
     integer(c_int) function dense_step(h, current_time, dt) bind(C, name="dense_step")

@@ -100,6 +100,17 @@ int dense_set_state(dense_handle h, const double* cons);
  * read_WriteAndLoad.f90:1-28). */
 int dense_get_state(dense_handle h, double* cons, double* prim);
 
+/* Download cons(ni+8,nj+8,4) as the reference's array holds it when it writes
+ * a .dat2d dump (write_date_binary, read_WriteAndLoad.f90:1-28, called at
+ * general_main.f90:43-44): interior = the current state; ghost layers = the
+ * reference's last boundary pass, i.e. the boundary rules applied to the last
+ * stage's input state at t^n (boundary_general before time_rk3_general(stages),
+ * general_main.f90:26-31); corner cells zero.  Before the first step after
+ * dense_set_state or dense_autotune, the ghost layers of the current state at
+ * the current time (as dense_get_state).  Stages = 1 keeps no copy of that
+ * input: DENSE_ERR_UNSUPPORTED after a step. */
+int dense_get_dump(dense_handle h, double* cons);
+
 /* One time step: `stages` x {get_primary, boundary_general,
  * flux_splitting_general, time_rk3_general(irk)}
  * (general_main.f90:26-31).  current_time is t^n, used by time-dependent
@@ -112,11 +123,23 @@ int dense_step(dense_handle h, double current_time, double* dt);
  * graphs; results are the same.) */
 
 /* Up to max_steps steps while (t < total_time && step < total_step)
- * (general_main.f90:23-24), replayed from a captured HIP graph with the clock
- * on the device.  On return *t and *step hold the advanced clock; dt_hist,
- * if not NULL, receives the dt of each step taken (max_steps entries). */
+ * (general_main.f90:23-24), replayed from captured HIP graphs (16-step chunks
+ * and one graph of the remainder) with the clock on the device.  On return *t
+ * and *step hold the advanced clock; dt_hist, if not NULL, must hold max_steps
+ * entries and receives the dt of each step taken (*steps_taken of them). */
 int dense_run(dense_handle h, int max_steps, double total_time, int total_step,
               double* t, int* step, double* dt_hist, int* steps_taken);
+
+/* dense_run that also stops at the reference's save clock: *elasp is
+ * elasp_time, advanced by dt after every step (general_main.f90:37), and the
+ * run ends after the first step at which *elasp >= dt_save (the save test of
+ * general_main.f90:40; at least one step runs, also when *elasp >= dt_save on
+ * entry), as well as at total_time, total_step or max_steps.
+ * A driver that replays the reference's loop calls it with max_steps = the
+ * steps to the next multiple of SAVE_STEP, then saves when either test holds.
+ * On return *elasp holds the advanced save clock. */
+int dense_run_saving(dense_handle h, int max_steps, double total_time, int total_step, double dt_save,
+                     double* elasp, double* t, int* step, double* dt_hist, int* steps_taken);
 
 /* GPU time of the last dense_run, from HIP events recorded on the handle's
  * stream around the graph launches, and the number of RK-stage kernel
@@ -134,7 +157,9 @@ int dense_update_params(dense_handle h, const dense_cfg* cfg);
 
 /* Time each tile shape (cfg.tile 4, 3, 1, 2, 5) on trial_steps steps of the
  * current state, restoring state and clock after every trial, and keep the
- * fastest as an explicit cfg.tile (dense_update_params keeps it).  *tile and
+ * fastest as an explicit cfg.tile (dense_update_params keeps it).  The trials
+ * overwrite the stage buffers: dense_get_theta and the dump ghost layers of
+ * dense_get_dump need a step after it.  *tile and
  * *best_ms (nullable) receive the choice and its trial time.  No counterpart in
  * the reference (one CPU code path); all shapes give the same bits. */
 int dense_autotune(dense_handle h, int trial_steps, int* tile, double* best_ms);
@@ -195,10 +220,16 @@ const char* dense_batch_last_error(dense_batch_handle b);
  *    rho = p = 1, u = mach sqrt(gamma); downstream the Rankine-Hugoniot state
  *    (shock_relation.py:7-13); upstream a compressible vortex (eps, rc, alpha)
  *    centred at (xc, yc) as in init_shockvortex_general.f90:34-51.
+ *  - DENSE_IC_SHOCK_VORTEX_REF: the reference's own benchmark-9 state,
+ *    init_shockvortex_general.f90:34-56 operation for operation (its
+ *    single-precision literals widened, Mach 1.1, the printed post-shock
+ *    constants 1.169082, 1.113299, 0, 1.245); the descriptor's parameters are
+ *    not read.  Equal to the reference's state bit for bit wherever the device
+ *    exp and pow round as the host's do.
  * Cell centres are ((i + 1/2) dx, (j + 1/2) dy) with the instance's metrics.
  * Every instance's clock is reset to 0.  Replaces dense_batch_set_state for
  * all instances at once; no host state crosses the bus. */
-enum { DENSE_IC_VORTEX = 1, DENSE_IC_SHOCK_VORTEX = 2 };
+enum { DENSE_IC_VORTEX = 1, DENSE_IC_SHOCK_VORTEX = 2, DENSE_IC_SHOCK_VORTEX_REF = 3 };
 typedef struct dense_ic {
     int kind;            /* DENSE_IC_VORTEX or DENSE_IC_SHOCK_VORTEX */
     double eps;          /* vortex strength */

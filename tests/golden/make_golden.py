@@ -9,6 +9,7 @@ the per-step dt it printed.  These files are data (inputs = the switches in
 container, where the reference sources exist.
 
     python tests/golden/make_golden.py [case names]   # all cases, or only those named
+    python tests/golden/make_golden.py --dumps [names] # whole dumps of DUMP_CASES
 """
 import json
 import os
@@ -61,6 +62,25 @@ CASES += [
 ]
 
 
+# Cases whose whole dump (ghost layers and corners included) is also stored:
+# the reference's cons array at a save holds the ghost values of its last
+# boundary pass (the last stage's input at t^n), general_main.f90:26-31,43-44.
+DUMP_CASES = ["ivc_101_253", "ivc_101_255_s2", "dmr_104", "rt_105", "sv_101_255", "ssl_101_255",
+              "sedov_101_255", "rm_104"]
+
+
+def write_dumps(only=None):
+    for name, bm, ni, nj, sp, rc, cfl, steps, extra in CASES:
+        if name not in DUMP_CASES or (only and name not in only):
+            continue
+        nml = dict(benchmark=bm, ni=ni, nj=nj, splitting=sp, recon=rc, cfl=cfl)
+        nml.update(extra)
+        d = po.run_reference(steps=steps, **nml)
+        np.savez_compressed(os.path.join(HERE, f"dump_{name}.npz"), cons=d["cons"], step=np.int64(d["step"]),
+                            time=np.float64(d["time"]))
+        print("dump", name, d["cons"].shape)
+
+
 def main(only=None):
     """only: names of the cases to (re)write; the index lists every case."""
     index = {}
@@ -80,4 +100,7 @@ def main(only=None):
 
 
 if __name__ == "__main__":
-    main(set(sys.argv[1:]) or None)
+    if sys.argv[1:2] == ["--dumps"]:
+        write_dumps(set(sys.argv[2:]) or None)
+    else:
+        main(set(sys.argv[1:]) or None)

@@ -157,7 +157,7 @@ def test_device_initial_states_reject_bad_descriptors():
     cfgs, _, insts = B.make_ensemble(n=2, seed=1, nmin=16, nmax=24, host_states=False)
     with B.Batch(cfgs) as b:
         good = [B.ic_desc(x) for x in insts]
-        for field, value in (("kind", 3), ("mach", 0.5), ("rc", 0.0), ("eps", float("nan"))):
+        for field, value in (("kind", 7), ("mach", 0.5), ("rc", 0.0), ("eps", float("nan"))):
             bad = [B.ic_desc(x) for x in insts]
             k = 1 if field in ("mach", "rc") else 0
             setattr(bad[k], field, value)

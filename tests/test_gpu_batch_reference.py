@@ -1,0 +1,148 @@
+"""Batched mode against the reference itself, not only against the GPU's
+single-block path.
+
+* The reference's own canned problems -- benchmark 1 (vortex, extrapolated x
+  sides), 2 (Sedov), 3 (Rayleigh-Taylor with gravity, gamma 5/3), 4
+  (Richtmyer-Meshkov), 5 (double Mach) and 9 (shock/vortex) with their
+  boundary routines -- sit in one Batch beside seeded ensemble instances and
+  advance together, one launch per RK stage.  Each reference instance must
+  equal the fixture the reference solver wrote (tests/golden), state and dt
+  sequence, bit for bit; each ensemble instance must equal the C oracle run
+  with the same metrics and boundary kinds, bit for bit.
+* DENSE_IC_SHOCK_VORTEX_REF generates the reference's benchmark-9 initial
+  state (init_shockvortex_general.f90:34-56) on the device: equal to the
+  oracle's bit for bit wherever exp and pow are not involved (the post-shock
+  region), within 1e-14 normwise elsewhere; ten batched steps from it stay
+  within the shocked-flow tolerance of the reference's own run (density L1
+  <= 1e-9).
+"""
+import importlib
+
+import numpy as np
+import pytest
+
+from conftest import dense, golden, golden_cases
+
+pytestmark = pytest.mark.gpu
+CASES = golden_cases()
+
+
+def _B():
+    return importlib.import_module("2dense_amd.batch")
+
+
+def _ref_cfg(name):
+    D = dense()
+    c = CASES[name]
+    return D.make_cfg(c["ni"], c["nj"], benchmark=c["benchmark"], splitting=c["splitting"], recon=c["recon"],
+                      cfl=c["cfl"], stages=c.get("stages", 3))
+
+
+def _oracle_of_cfg(oracle, cfg, cons):
+    """The C oracle on the metrics, boundary kinds, gamma and body force of a
+    dense_cfg, from the interior state cons."""
+    p = oracle.problem_params(1, cfg.ni, cfg.nj, splitting=cfg.splitting, recon=cfg.recon, cfl=cfg.cfl,
+                              gamma=cfg.gamma, delta=cfg.delta, stages=cfg.stages)
+    p.dx, p.dy = cfg.dx, cfg.dy
+    p.if_bodyforce = cfg.if_bodyforce
+    for s in range(4):
+        p.bc[s] = cfg.bc[s]
+        for v in range(4):
+            p.bc_state[s][v] = cfg.bc_state[s][v]
+    o = oracle.Oracle(p)
+    pad = np.zeros(o.shape_cons)
+    pad[:, 4:-4, 4:-4] = cons
+    o.set_cons(pad)
+    return o
+
+
+GROUPS = {
+    "101/255": ["ivc_101_255", "sv_101_255", "rt_101_255", "sedov_101_255"],
+    "104": ["ivc_104", "sv_104", "dmr_104", "rm_104", "sedov_104", "rt_104"],
+    "105": ["ivc_105", "sv_105", "dmr_105", "rm_105", "rt_105"],
+}
+
+
+@pytest.mark.parametrize("group", sorted(GROUPS))
+def test_reference_instances_in_a_batch_equal_the_reference(group, oracle):
+    B = _B()
+    names = GROUPS[group]
+    sp, rc = CASES[names[0]]["splitting"], CASES[names[0]]["recon"]
+    ens_cfgs, ens_states, _ = B.make_ensemble(n=3, seed=17, nmin=16, nmax=40, splitting=sp, recon=rc)
+    cfgs = [_ref_cfg(n) for n in names] + list(ens_cfgs)
+    states = []
+    for n in names:
+        c = CASES[n]
+        o = oracle.Oracle(oracle.problem_params(c["benchmark"], c["ni"], c["nj"], splitting=sp, recon=rc,
+                                                cfl=c["cfl"]))
+        o.init_state(c["benchmark"])
+        states.append(o.cons()[:, 4:-4, 4:-4].copy())
+    states += list(ens_states)
+    print(f"batch {group}: reference {names} + ensemble {[(c.ni, c.nj) for c in ens_cfgs]}", flush=True)
+    # the oracle on each ensemble instance, same metrics and boundary kinds
+    ens_oracles = [_oracle_of_cfg(oracle, cfg, st) for cfg, st in zip(ens_cfgs, ens_states)]
+    nref = len(names)
+    with B.Batch(cfgs) as b:
+        for i, st in enumerate(states):
+            b.set_state(i, st)
+        done, hist_all = 0, []
+        for target in sorted({CASES[n]["steps"] for n in names}):
+            hist_all.append(b.run(target - done, want_dts=True))
+            done = target
+            hist = np.concatenate(hist_all)
+            for i, n in enumerate(names):
+                if CASES[n]["steps"] != done:
+                    continue
+                g = golden(n)
+                got = b.interior(i)
+                assert np.array_equal(got, g["cons"]), f"{n}: max |diff| {np.max(np.abs(got - g['cons']))}"
+                assert np.array_equal(hist[:, i], g["dts"]), f"{n}: dt sequence"
+                assert b.t[i] == float(g["time"])
+            for k, o in enumerate(ens_oracles):
+                while o.current_step < done:
+                    o.step()
+                got = b.interior(nref + k)
+                assert np.array_equal(got, o.interior()), f"ensemble instance {k}: max |diff| " \
+                                                          f"{np.max(np.abs(got - o.interior()))}"
+
+
+def test_device_reference_shock_vortex_initial_state(oracle):
+    """The reference's benchmark-9 state generated on the device, next to a
+    seeded vortex instance, against the oracle's restatement of
+    init_shockvortex_general.f90 (itself pinned bitwise by the sv_* fixtures)."""
+    B = _B()
+    D = dense()
+    c = CASES["sv_101_255"]
+    cfg_ref = _ref_cfg("sv_101_255")
+    ens_cfgs, _, ens_insts = B.make_ensemble(n=1, seed=5, nmin=24, nmax=24, host_states=False)
+    o = oracle.Oracle(oracle.problem_params(9, c["ni"], c["nj"], splitting=101, recon=255, cfl=c["cfl"]))
+    o.init_state(9)
+    want = o.interior()
+    with B.Batch([cfg_ref] + list(ens_cfgs)) as b:
+        b.init_ic([B.Instance("shock_vortex_ref", c["ni"], c["nj"], {})] + list(ens_insts))
+        got = b.interior(0)
+        x = (np.arange(c["ni"], dtype=np.float32) + np.float32(0.5)).astype(np.float64) * cfg_ref.dx
+        post = np.broadcast_to(x >= 0.5, got.shape[1:])
+        # the post-shock constants and every cell whose exp/pow agree: bitwise
+        assert np.array_equal(got[:, post], want[:, post])
+        for k in range(4):
+            err = np.linalg.norm(got[k] - want[k]) / np.linalg.norm(want[k])
+            assert err <= 1e-14, f"component {k}: {err}"
+        same = np.all(got == want, axis=0)
+        print(f"device vs oracle benchmark-9 state: {same.sum()} of {same.size} cells bitwise", flush=True)
+        # ten batched steps from it against the reference's run from its own state
+        hist = b.run(c["steps"], want_dts=True)
+        g = golden("sv_101_255")
+        got = b.interior(0)
+        l1 = float(np.sum(np.abs(got[0] - g["cons"][0])) * cfg_ref.dx * cfg_ref.dy)
+        assert l1 <= 1e-9, l1
+        rel = np.max(np.abs(got - g["cons"])) / np.max(np.abs(g["cons"]))
+        assert rel <= 1e-11, rel
+        assert np.allclose(hist[:, 0], g["dts"], rtol=1e-13, atol=0)
+        if same.all():
+            assert np.array_equal(got, g["cons"]) and np.array_equal(hist[:, 0], g["dts"])
+    with pytest.raises(D.DenseError):  # descriptors are validated before any launch
+        with B.Batch([cfg_ref]) as b:
+            d = B.IcDesc()
+            d.kind = 7
+            b.init_ic([d])

@@ -46,6 +46,18 @@ def test_driver_matches_reference(tmp_path, name):
     assert np.array_equal(got, g["cons"]), f"max |diff| {np.max(np.abs(got - g['cons']))}"
     assert np.array_equal(dts, g["dts"])
     assert d["step"] == c["steps"] and d["time"] == float(g["time"])
+    if os.path.exists(os.path.join(ROOT, "tests", "golden", f"dump_{name}.npz")):
+        # the whole array: ghost layers of the last boundary pass, zero corners
+        assert np.array_equal(d["cons"], golden(f"dump_{name}")["cons"])
+    # the whole file, byte for byte, against the reference's dump of the same input.in
+    import pyoracle as po
+    if os.path.exists(po.REF_BIN):
+        ref_dir = tmp_path / "ref"
+        ref_dir.mkdir()
+        (ref_dir / "input.in").write_text((tmp_path / "input.in").read_text())
+        subprocess.run([po.REF_BIN], cwd=ref_dir, capture_output=True, check=True, timeout=120)
+        f = f"2d_{c['steps']}.dat2d"
+        assert (tmp_path / f).read_bytes() == (ref_dir / f).read_bytes()
 
 
 def test_driver_restart_roundtrip(tmp_path):

@@ -93,3 +93,27 @@ def test_gpu_run_stops_at_total_time(oracle, name, total):
         assert s.current_step == o.current_step
         assert s.current_time == o.time
         assert np.array_equal(s.interior(), o.interior())
+
+
+@pytest.mark.parametrize("sp,rc", [(101, 251), (102, 251), (103, 255), (104, 255), (105, 255)])
+def test_tile_shapes_agree_when_workgroups_take_many_tiles(sp, rc):
+    """Double Mach 1920x480: 3,840 tiles of 16x15 (1,920 of 32x15, 17,280 of
+    8x7) against the few hundred workgroups the persistent grid holds, so each
+    workgroup pulls many tiles from its XCD's queue and rewrites its LDS for
+    each: the 512-thread shapes whose face fluxes overwrite the cell records
+    (32x15 and 16x15/512 with upwind5), the x/y halves of 16x15/512, the
+    Steger-Warming bands and the characteristic faces all give the bits of
+    the 16x15 shape after 4 steps."""
+    D = dense()
+    bench = __import__("bench")
+    cons = bench.dmr_initial_state(1920, 480)
+    out = {}
+    for tile in (4, 1, 2, 3, 5):
+        print(f"DMR 1920x480 {sp}/{rc} tile {tile}", flush=True)
+        cfg = D.make_cfg(1920, 480, benchmark=5, splitting=sp, recon=rc, cfl=0.6, tile=tile)
+        with D.Solver(cfg) as s:
+            s.set_state(cons)
+            s.run(4)
+            out[tile] = s.interior()
+    for tile in (1, 2, 3, 5):
+        assert np.array_equal(out[tile], out[4]), f"tile {tile}: max |diff| {np.max(np.abs(out[tile] - out[4]))}"
