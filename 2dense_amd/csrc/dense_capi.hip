@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <atomic>
 #include <climits>
 #include <cstring>
 #include <new>
@@ -753,3 +754,154 @@ extern "C" int dense_sync(dense_handle h)
 
 extern "C" const char* dense_last_error(dense_handle h) { return h ? h->err : g_err; }
 
+// ---------------------------------------------------------------------------
+// Allocation-free stepping (dense_run_noalloc).  The scratch memory of a run
+// -- the device clock and launch counters (DevState) and the two stage
+// buffers U^(1), U^(2) -- lives in one file-scope device array instead of
+// memory the library allocates.  Host code never passes `scratch` itself to a
+// kernel (the name is a host-side stand-in): its device address is looked up
+// once per device with hipGetSymbolAddress and kept in g_scratch[device].
+// Every call on a device shares that device's array and leaves its contents
+// behind; each call clears the clock and counters it needs with a kernel, and
+// calls that overlap on one device race.
+namespace {
+constexpr long long NOALLOC_MAX_CELLS = 1ll << 20;  // ld * nj per stage buffer
+constexpr long long NOALLOC_STATE_DOUBLES = 512;    // 4 KB for the DevState
+static_assert(sizeof(DevState) <= NOALLOC_STATE_DOUBLES * sizeof(double), "DevState fits its slot");
+constexpr long long NOALLOC_DOUBLES = NOALLOC_STATE_DOUBLES + 2 * 4 * NOALLOC_MAX_CELLS;  // 64 MiB + 4 KB
+constexpr int NOALLOC_MAX_DEVICES = 64;
+}  // namespace
+
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace {
+std::atomic<double*> g_scratch[NOALLOC_MAX_DEVICES];
+
+// Device address of the current device's scratch array (looked up on the
+// first call on that device).
+hipError_t scratch_ptr(double** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+    double* p = g_scratch[dev].load(std::memory_order_acquire);
+    if (!p) {
+        e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[dev].store(p, std::memory_order_release);
+    }
+    *out = p;
+    return hipSuccess;
+}
+
+// A fresh DevState: reduction ring, sequence number, launch counters, error
+// flag and history count zero (a kernel, not a memset, so that a caller may
+// capture the call in a graph).
+__global__ void noalloc_reset_kernel(DevState* st)
+{
+    for (int s = 0; s < 4; s++) st->alpha[s][0] = st->alpha[s][1] = 0ull;
+    for (int x = 0; x < 8; x++) st->qnext[x].v = 0u;
+    st->blocks_done.v = 0u;
+    st->seq = 0;
+    st->error = 0;
+    st->active = 0;
+    st->hist_n = 0;
+    st->hist_cap = 0;
+    st->dt = 0.0;
+    st->t_prev = 0.0;
+}
+}  // namespace
+
+extern "C" int dense_run_noalloc(const dense_cfg* cfg, double* u, int ld, int nsteps, double t0, void* stream,
+                                 double* t_out, double* dt_out)
+{
+    if (!cfg || !u || nsteps < 0) return fail(nullptr, DENSE_ERR_ARG, "bad arguments");
+    dense_cfg c = *cfg;
+    if (c.benchmark != 0) {
+        dense_cfg d = c;
+        int rc = dense_problem_defaults(c.benchmark, c.ni, c.nj, &d);
+        if (rc) return rc;
+        for (int s = 0; s < 4; s++) {
+            c.bc[s] = d.bc[s];
+            for (int v = 0; v < 4; v++) c.bc_state[s][v] = d.bc_state[s][v];
+        }
+        if (!(c.dx > 0)) c.dx = d.dx;
+        if (!(c.dy > 0)) c.dy = d.dy;
+    }
+    char why[256];
+    int rc = validate_cfg(&c, why, sizeof why);
+    if (rc) return fail(nullptr, rc, "%s", why);
+    if (c.bc[DENSE_SIDE_LEFT] == DENSE_BC_SSL_INLET)
+        return fail(nullptr, DENSE_ERR_UNSUPPORTED, "the shear-layer inflow steps from the host (use dense_run)");
+    if (ld < c.ni) return fail(nullptr, DENSE_ERR_ARG, "row pitch %d below ni = %d", ld, c.ni);
+    const long long plane = (long long)ld * c.nj;
+    if (plane > NOALLOC_MAX_CELLS)
+        return fail(nullptr, DENSE_ERR_ARG,
+                    "%lld cells per plane (ld %d x nj %d) exceed the allocation-free limit of %lld; use dense_run",
+                    plane, ld, c.nj, NOALLOC_MAX_CELLS);
+    if (nsteps == 0) {
+        if (t_out) *t_out = t0;
+        if (dt_out) *dt_out = 0.0;
+        return DENSE_OK;
+    }
+    double* base = nullptr;
+    hipError_t e = scratch_ptr(&base);
+    if (e != hipSuccess) return fail(nullptr, DENSE_ERR_HIP, "scratch address: %s", hipGetErrorString(e));
+    DevState* st = reinterpret_cast<DevState*>(base);
+    double* U1 = base + NOALLOC_STATE_DOUBLES;
+    double* U2 = U1 + 4 * plane;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+
+    StageArgs a{};
+    a.st = st;
+    a.dt_hist = nullptr;
+    a.ni = c.ni;
+    a.nj = c.nj;
+    a.ld = ld;
+    a.plane = plane;
+    a.dx = c.dx;
+    a.dy = c.dy;
+    a.gamma = c.gamma;
+    a.cfl = c.cfl;
+    a.delta = c.delta;
+    a.dt_const = c.dt;
+    a.sqrt3 = std::sqrt(3.0);
+    a.if_const_dt = c.if_const_dt;
+    a.if_bodyforce = c.if_bodyforce;
+    const int tile = resolve_tile(c.tile, c.splitting, (long long)c.ni * c.nj);
+    a.ntiles = tile_count(tile, c.ni, c.nj, &a.tiles_x);
+    for (int k = 0; k < 4; k++) {
+        a.bc.kind[k] = c.bc[k];
+        for (int v = 0; v < 4; v++) a.bc.state[k][v] = c.bc_state[k][v];
+    }
+    a.bc_rows = nullptr;
+    const int scheme = scheme_of(c.splitting, c.recon);
+
+    hipLaunchKernelGGL(noalloc_reset_kernel, dim3(1), dim3(1), 0, s, st);
+    HIPCHK(nullptr, hipGetLastError());
+    a.us = u;
+    HIPCHK(nullptr, launch_max(a, s));  // wave speeds of the input state
+    HIPCHK(nullptr, launch_set_clock(st, t0, INFINITY, 0, nsteps, s));
+    double* Us[3] = {u, U1, U2};
+    const int S = c.stages;
+    for (int n = 0; n < nsteps; n++) {
+        for (int k = 1; k <= S; k++) {
+            a.stage = k;
+            a.last_stage = (k == S);
+            a.un = u;
+            a.us = k == 1 ? u : Us[k - 1];
+            a.out = (k == S && S > 1) ? u : Us[k];
+            HIPCHK(nullptr, launch_stage(scheme, tile, a, s));
+        }
+        if (S == 1) HIPCHK(nullptr, hipMemcpyAsync(u, U1, sizeof(double) * 4 * plane, hipMemcpyDeviceToDevice, s));
+    }
+    if (!t_out && !dt_out) return DENSE_OK;  // enqueued only
+    DevState h{};
+    HIPCHK(nullptr, hipMemcpyAsync(&h, st, sizeof h, hipMemcpyDeviceToHost, s));
+    HIPCHK(nullptr, hipStreamSynchronize(s));
+    if (t_out) *t_out = h.t;
+    if (dt_out) *dt_out = h.dt;
+    if (h.error) return fail(nullptr, DENSE_ERR_NONFINITE, "non-finite wave speed by step %d", h.step);
+    return DENSE_OK;
+}

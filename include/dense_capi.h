@@ -141,6 +141,30 @@ int dense_run(dense_handle h, int max_steps, double total_time, int total_step,
 int dense_run_saving(dense_handle h, int max_steps, double total_time, int total_step, double dt_save,
                      double* elasp, double* t, int* step, double* dt_hist, int* steps_taken);
 
+/* Allocation-free variant of dense_run.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  It allocates
+ * nothing: the run's scratch memory (device clock, launch counters, the two
+ * stage buffers) is a fixed device array of the library (64 MiB + 4 KB per
+ * GPU, held until the process ends), and no handle is created.
+ *  u      DEVICE pointer to the block's interior state: 4 planes (rho, rho u,
+ *         rho v, E) of nj rows with row pitch ld doubles (ld >= ni), advanced
+ *         in place by nsteps steps (no early stop), each with its CFL step
+ *         (or cfg->dt when if_const_dt) and boundary time t0 + the steps so far;
+ *  stream a hipStream_t, or NULL for the null stream;
+ *  t_out, dt_out  host pointers (nullable).  If either is given the call waits
+ *         for the stream and returns the advanced clock and the last step size
+ *         (and DENSE_ERR_NONFINITE for a blown-up state); if both are NULL it
+ *         only enqueues kernels on the stream, so a caller may capture it.
+ * Limit: ld * nj <= 1,048,576 (DMR 1920x480 fits); larger blocks return
+ * DENSE_ERR_ARG.  The shear-layer inflow (host-evaluated clock terms) returns
+ * DENSE_ERR_UNSUPPORTED.  Every call on one GPU shares that GPU's scratch and
+ * leaves its contents behind: calls that overlap on one GPU (two host threads,
+ * or two streams in flight) race.  Results are bit for bit those of dense_run
+ * on the same state. */
+int dense_run_noalloc(const dense_cfg* cfg, double* u, int ld, int nsteps, double t0, void* stream,
+                      double* t_out, double* dt_out);
+
 /* GPU time of the last dense_run, from HIP events recorded on the handle's
  * stream around the graph launches, and the number of RK-stage kernel
  * launches that did work in it.  No reference counterpart (the reference
