@@ -1,8 +1,8 @@
 """Build the native parts of the package in-tree.
 
 * ``lib2dense_amd.so`` -- the HIP library behind include/dense_capi.h, built
-  for gfx950 with FMA contraction off (the kernels keep the reference's
-  FP64 rounding sequence) and packed-FP32 instructions disabled.
+  for gfx950 with FMA contraction on (IEEE semantics otherwise: no fast-math,
+  so NaN and infinity checks stay) and packed-FP32 instructions disabled.
 * ``bin/dense_gpu`` -- the Fortran driver (iso_c_binding over the C ABI),
   built with AMD flang when it is installed.
 
@@ -30,7 +30,7 @@ ARCH = os.environ.get("DENSE_OFFLOAD_ARCH", "gfx950")
 HIP_SOURCES = ["dense_kernels.hip", "dense_batch.hip", "dense_capi.hip", "dense_adjoint.hip"]
 HIP_FLAGS = [
     "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
-    "-ffp-contract=off", "-fno-fast-math",
+    "-ffp-contract=on", "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]

@@ -549,6 +549,7 @@ __device__ __forceinline__ double floor_mod(double a, double b)
 
 __device__ __forceinline__ void cons_of(double rho, double u, double v, double p, double gamma, double* q)
 {
+#pragma clang fp contract(off)  // initial states as the host evaluates them
     q[0] = rho;
     q[1] = rho * u;
     q[2] = rho * v;
@@ -557,6 +558,7 @@ __device__ __forceinline__ void cons_of(double rho, double u, double v, double p
 
 __device__ __forceinline__ void ic_vortex(const dense_ic& P, double gamma, double x, double y, double* q)
 {
+#pragma clang fp contract(off)  // initial states as the host evaluates them
     constexpr double pi = 3.141592653589793;
     const double L = P.length;
     const double dxp = floor_mod(x - P.xc + 0.5 * L, L) - 0.5 * L;
@@ -573,6 +575,7 @@ __device__ __forceinline__ void ic_vortex(const dense_ic& P, double gamma, doubl
 
 __device__ __forceinline__ void ic_shock_vortex(const dense_ic& P, double gamma, double x, double y, double* q)
 {
+#pragma clang fp contract(off)  // initial states as the host evaluates them
     const double sg = sqrt(gamma);
     const double m2 = P.mach * P.mach;
     if (x < P.xs) {
@@ -604,6 +607,7 @@ __device__ __forceinline__ void ic_shock_vortex(const dense_ic& P, double gamma,
 // (i, j) are 0-based interior indices.
 __device__ __forceinline__ void ic_shock_vortex_ref(double gamma, int i, int j, double dx, double dy, double* q)
 {
+#pragma clang fp contract(off)  // initial states as the host evaluates them
     const double x_c = (double)0.25f, y_c = (double)0.5f;
     const double epsl = (double)0.3f, r_c = (double)0.05f, alpha = (double)0.204f;
     const double x = (double)((float)(i + 1) - 0.5f) * dx;

@@ -1,8 +1,9 @@
 // Device-side numerics of the explicit update: primitive recovery, ghost
 // cells, flux splitting, 5th-order reconstructions and the characteristic
-// projection.  Every expression keeps the reference's evaluation order and
-// the library is built with -ffp-contract=off, so results are bitwise equal
-// to the reference's FP64 arithmetic.  Citations are relative to the
+// projection, in FP64.  The formulas are the reference's; divisions in the
+// per-face and per-cell paths go through refined reciprocals and the library
+// is built with FMA contraction, so results agree with the reference's to
+// rounding level rather than bit for bit.  Citations are relative to the
 // reference repository root.
 #pragma once
 #include "dense_internal.h"
@@ -23,20 +24,18 @@ constexpr double C30 = 0.1, C31 = 0.6, C32 = 0.3;
 enum Recon : int { R_UP5 = 0, R_JS = 1, R_Z = 2 };
 
 // ---------------------------------------------------------------- division
-// gfx950 has no f64 divide; the compiler expands a/b into
-//   s0 = div_scale(b, b, a); y = rcp(s0) refined by two Newton steps
-//   (e = fma(-s0, y, 1), y = fma(y, e, y), twice); s1 = div_scale(a, b, a);
-//   q0 = s1*y; r = fma(-s0, q0, s1); q = div_fmas(r, y, q0); div_fixup(q, b, a).
-// Both div_scale steps return their operand unscaled and leave the scale flag
-// clear, and div_fixup returns its input, when the operands and the quotient
-// are normal numbers away from the exponent limits:
-//   2^-969 <= |a|, b normal with |b| < 2^1021, 2^-1022 <= |a/b| < 2^767,
-// or a = +-0 with b > 0 (div_fixup returns a zero of a's sign).  In that range
-// a/b is, bit for bit, y = nr_recip(b); q = div_by(a, b, y), so a divisor
-// shared by several quotients needs one reciprocal.  Every use below tests a
-// sufficient range condition per quotient and recomputes that quotient with
-// '/' where it fails (a short divergent branch, taken on extreme data only).
-__device__ __forceinline__ double nr_recip(double b)
+// gfx950 has no f64 divide: an IEEE a/b expands to 11 instructions around a
+// quarter-rate v_rcp_f64 (div_scale twice, two Newton steps, div_fmas,
+// div_fixup).  The hot paths divide through reciprocals instead, and the
+// library is compiled with FMA contraction: results agree with the
+// reference's step-by-step IEEE evaluation to rounding level (the tolerance
+// the workloads state is 1e-11 relative on smooth flows and 1e-9 density L1
+// on shocked flows), not bit for bit.
+//   rcp(b): v_rcp_f64 refined by two Newton steps (within 1 ulp of 1/b for
+//           normal b whose reciprocal is normal).
+//   fdiv(a, b): a*rcp(b) corrected by one residual step (the quotient to
+//           within an ulp for normal operands away from the exponent limits).
+__device__ __forceinline__ double rcp(double b)
 {
     const double r = __builtin_amdgcn_rcp(b);
     const double e0 = fma(-b, r, 1.0);
@@ -45,62 +44,30 @@ __device__ __forceinline__ double nr_recip(double b)
     return fma(y1, e1, y1);
 }
 
-// The residual is formed negated, fma(b, q0, -a) = -fma(-b, q0, a) exactly,
-// so that a = -0 yields -0 like IEEE division (identical bits otherwise).
-__device__ __forceinline__ double div_by(double a, double b, double y)
+__device__ __forceinline__ double fdiv(double a, double b)
 {
+    const double y = rcp(b);
     const double q0 = a * y;
-    const double rn = fma(b, q0, -a);
-    return fma(-rn, y, q0);
-}
-
-// n/12 for n >= 0 (the smoothness indicators).  The residual is formed at
-// scale 2^128: for 2^-969 <= n <= 2^890 this is the sequence above with an
-// exact residual scaled by a power of two (identical bits); for tiny n
-// (2^-1018 <= n < 2^-969, where the compiler's sequence scales both operands
-// by 2^128) it is that scaled sequence, q = RN(n*2^128/12)*2^-128 = RN(n/12),
-// the quotient being normal.  n = 0 gives +0.  Other n take '/'.
-// 12 is hidden from constant folding so that its reciprocal is formed at run
-// time by the same instructions as in the compiler's n/12.0.
-__device__ __forceinline__ double div12(double n)
-{
-    double twelve = 12.0;
-    asm("" : "+v"(twelve));
-    const double y = nr_recip(twelve);
-    const double q0 = n * y;
-    const double rn = fma(twelve * 0x1p128, q0, -(n * 0x1p128));
-    double q = fma(-rn, y * 0x1p-128, q0);
-    if (__builtin_expect(!(n == 0.0 || (n >= 0x1p-1018 && n <= 0x1p890)), 0)) q = n / 12.0;
-    return q;
+    return fma(fma(-b, q0, a), y, q0);
 }
 
 // ---------------------------------------------------------------- stencils
 
-// The stencils take FAST = true in the characteristic WENO-Z scheme (divisions
-// are about half of its instructions; measured 3-4 % faster per stage) and
-// FAST = false ('/') elsewhere: the component-wise schemes run at the register
-// limit of their occupancy, and AdaWENO measured no gain.
-template <bool FAST>
-__device__ __forceinline__ double over12(double n)
-{
-    if constexpr (FAST) return div12(n);
-    else return n / 12.0;
-}
-
-// smoothness indicators, weno_mod.f90:152-159 / :175-182
-template <bool FAST>
+// smoothness indicators, weno_mod.f90:152-159 / :175-182:
+// beta = 13/12 x^2 + 1/4 y^2
 __device__ __forceinline__ void smooth_ind(const double* lf, double& is0, double& is1, double& is2)
 {
+    constexpr double C13 = 13.0 / 12.0;
     double x, y;
     x = (lf[0] - 2.0 * lf[1]) + lf[2];
     y = (lf[0] - 4.0 * lf[1]) + 3.0 * lf[2];
-    is0 = over12<FAST>(13.0 * (x * x)) + (y * y) / 4.0;
+    is0 = C13 * (x * x) + 0.25 * (y * y);
     x = (lf[1] - 2.0 * lf[2]) + lf[3];
     y = lf[1] - lf[3];
-    is1 = over12<FAST>(13.0 * (x * x)) + (y * y) / 4.0;
+    is1 = C13 * (x * x) + 0.25 * (y * y);
     x = (lf[2] - 2.0 * lf[3]) + lf[4];
     y = (3.0 * lf[2] - 4.0 * lf[3]) + lf[4];
-    is2 = over12<FAST>(13.0 * (x * x)) + (y * y) / 4.0;
+    is2 = C13 * (x * x) + 0.25 * (y * y);
 }
 
 // candidate polynomials, weno_mod.f90:57-59
@@ -111,110 +78,72 @@ __device__ __forceinline__ void candidates(const double* lf, double& q0, double&
     q2 = A320 * lf[2] + A321 * lf[3] + A322 * lf[4];
 }
 
-// w_k = a_k / s for a_k = c_k(...) >= c_k >= 0.1 and s = sum a_k: one shared
-// reciprocal when s <= 2^1000 (then 0.1 <= a_k <= s and the quotients lie in
-// [2^-1004, 1]); '/' otherwise (s infinite or NaN).
-template <bool FAST>
-__device__ __forceinline__ void normalize3(double aa0, double aa1, double aa2, double s, double& w0, double& w1,
-                                           double& w2)
-{
-    if constexpr (!FAST) {
-        w0 = aa0 / s;
-        w1 = aa1 / s;
-        w2 = aa2 / s;
-        return;
-    }
-    const double ys = nr_recip(s);
-    w0 = div_by(aa0, s, ys);
-    w1 = div_by(aa1, s, ys);
-    w2 = div_by(aa2, s, ys);
-    if (__builtin_expect(!(s <= 0x1p1000), 0)) {
-        w0 = aa0 / s;
-        w1 = aa1 / s;
-        w2 = aa2 / s;
-    }
-}
-
-// WENO-Z non-linear weights (+ AdaWENO indicator theta), weno_mod.f90:67-95,168-188
-template <bool FAST>
-__device__ __forceinline__ void weno5z_omega(const double* lf, double& w0, double& w1, double& w2, double* theta)
+// WENO-Z unnormalised weights a_k = c_k (1 + (tau5/(eps + beta_k))^2),
+// tau5 = |beta_2 - beta_0| (weno_mod.f90:184-188).
+__device__ __forceinline__ void weno5z_alpha(const double* lf, double& aa0, double& aa1, double& aa2)
 {
     double is0, is1, is2;
-    smooth_ind<FAST>(lf, is0, is1, is2);
+    smooth_ind(lf, is0, is1, is2);
     const double tau5 = fabs(is2 - is0);
-    double t;
-    t = tau5 / (WENO_SS + is0);
-    const double aa0 = C30 * (1.0 + t * t);
-    t = tau5 / (WENO_SS + is1);
-    const double aa1 = C31 * (1.0 + t * t);
-    t = tau5 / (WENO_SS + is2);
-    const double aa2 = C32 * (1.0 + t * t);
+    const double t0 = tau5 * rcp(WENO_SS + is0);
+    const double t1 = tau5 * rcp(WENO_SS + is1);
+    const double t2 = tau5 * rcp(WENO_SS + is2);
+    aa0 = C30 + C30 * (t0 * t0);
+    aa1 = C31 + C31 * (t1 * t1);
+    aa2 = C32 + C32 * (t2 * t2);
+}
+
+// WENO-Z normalised weights w_k = a_k / sum a and AdaWENO's indicator
+// theta = 1/(1 + (sum a - 1)^2) (weno_mod.f90:67-95)
+__device__ __forceinline__ void weno5z_omega(const double* lf, double& w0, double& w1, double& w2, double* theta)
+{
+    double aa0, aa1, aa2;
+    weno5z_alpha(lf, aa0, aa1, aa2);
     const double s = aa0 + aa1 + aa2;
-    normalize3<FAST>(aa0, aa1, aa2, s, w0, w1, w2);
+    const double ys = rcp(s);
+    w0 = aa0 * ys;
+    w1 = aa1 * ys;
+    w2 = aa2 * ys;
     if (theta) {
         const double d = s - 1.0;
-        *theta = 1.0 / (1.0 + d * d);
+        *theta = rcp(1.0 + d * d);
     }
 }
 
-// weno5z_stencil, weno_mod.f90:50-65
-template <bool FAST>
+// weno5z_stencil, weno_mod.f90:50-65: sum w_k q_k = (sum a_k q_k) / sum a_k,
+// one reciprocal per stencil
 __device__ __forceinline__ double weno5z(const double* lf)
 {
-    double q0, q1, q2, w0, w1, w2;
+    double q0, q1, q2, aa0, aa1, aa2;
     candidates(lf, q0, q1, q2);
-    weno5z_omega<FAST>(lf, w0, w1, w2, nullptr);
-    return w0 * q0 + w1 * q1 + w2 * q2;
+    weno5z_alpha(lf, aa0, aa1, aa2);
+    return (aa0 * q0 + aa1 * q1 + aa2 * q2) * rcp(aa0 + aa1 + aa2);
 }
 
-// weno5js_stencil, weno_mod.f90:98-165.  FAST: a_k = c_k/(eps+beta_k)^2 through
-// reciprocals while (eps+beta_k)^2 <= 2^900 (then a_k in [2^-904, 2^40] and
-// sum <= 2^42), '/' otherwise.
-template <bool FAST>
+// weno5js_stencil, weno_mod.f90:98-165: a_k = c_k/(eps+beta_k)^2
 __device__ __forceinline__ double weno5js(const double* lf)
 {
-    double q0, q1, q2, is0, is1, is2, s;
+    double q0, q1, q2, is0, is1, is2;
     candidates(lf, q0, q1, q2);
-    smooth_ind<FAST>(lf, is0, is1, is2);
-    s = WENO_SS + is0;
-    const double d0 = s * s;
-    s = WENO_SS + is1;
-    const double d1 = s * s;
-    s = WENO_SS + is2;
-    const double d2 = s * s;
-    double aa0, aa1, aa2;
-    if constexpr (FAST) {
-        aa0 = div_by(C30, d0, nr_recip(d0));
-        aa1 = div_by(C31, d1, nr_recip(d1));
-        aa2 = div_by(C32, d2, nr_recip(d2));
-        if (__builtin_expect(!(d0 <= 0x1p900 && d1 <= 0x1p900 && d2 <= 0x1p900), 0)) {
-            aa0 = C30 / d0;
-            aa1 = C31 / d1;
-            aa2 = C32 / d2;
-        }
-    } else {
-        aa0 = C30 / d0;
-        aa1 = C31 / d1;
-        aa2 = C32 / d2;
-    }
-    const double sum = aa0 + aa1 + aa2;
-    double w0, w1, w2;
-    normalize3<FAST>(aa0, aa1, aa2, sum, w0, w1, w2);
-    return w0 * q0 + w1 * q1 + w2 * q2;
+    smooth_ind(lf, is0, is1, is2);
+    const double r0 = rcp(WENO_SS + is0), r1 = rcp(WENO_SS + is1), r2 = rcp(WENO_SS + is2);
+    const double aa0 = C30 * (r0 * r0), aa1 = C31 * (r1 * r1), aa2 = C32 * (r2 * r2);
+    return (aa0 * q0 + aa1 * q1 + aa2 * q2) * rcp(aa0 + aa1 + aa2);
 }
 
-// scheme_upwind5_general.f90:86-90
+// scheme_upwind5_general.f90:86-90: (2, -13, 47, 27, -3) / 60
 __device__ __forceinline__ double upwind5(const double* lf)
 {
-    return (2.0 * lf[0] - 13.0 * lf[1] + 47.0 * lf[2] + 27.0 * lf[3] - 3.0 * lf[4]) / 60.0;
+    constexpr double R60 = 1.0 / 60.0;
+    return (2.0 * lf[0] - 13.0 * lf[1] + 47.0 * lf[2] + 27.0 * lf[3] - 3.0 * lf[4]) * R60;
 }
 
 template <int R>
 __device__ __forceinline__ double recon(const double* lf)
 {
     if constexpr (R == R_UP5) return upwind5(lf);
-    else if constexpr (R == R_JS) return weno5js<false>(lf);
-    else return weno5z<false>(lf);
+    else if constexpr (R == R_JS) return weno5js(lf);
+    else return weno5z(lf);
 }
 
 // ---------------------------------------------------------------- cells
@@ -226,14 +155,15 @@ template <bool WITH_C = true>
 __device__ __forceinline__ void prim_of(double gamma, const double* c, double* pr)
 {
     const double rho = c[0];
-    const double u = c[1] / rho;
-    const double v = c[2] / rho;
+    const double ir = rcp(rho);
+    const double u = c[1] * ir;
+    const double v = c[2] * ir;
     double p = (gamma - 1.0) * (c[3] - 0.5 * rho * (u * u + v * v));
     if (p < 0.0) p = P_FLOOR;
     pr[0] = u;
     pr[1] = v;
     pr[2] = p;
-    pr[3] = WITH_C ? sqrt(gamma * p / rho) : 0.0;
+    pr[3] = WITH_C ? sqrt(gamma * p * ir) : 0.0;
 }
 
 // Dirichlet ghost from a primitive state, boundary_doublemach_general.f90:52-61
@@ -385,31 +315,32 @@ __device__ __forceinline__ void sw_flux(double gamma, double rho, double u, doub
     const double un = DIR == 0 ? u : v;
     const double uu2 = u * u + v * v;
     const double lam[3] = {un, un - alpha, un + alpha};
+    const double g2 = 2.0 * (gamma - 1.0);
+    const double r2g = rho * rcp(2.0 * gamma);        // rho / (2 gamma)
+    const double wc = (3.0 - gamma) * (alpha * alpha) * rcp(g2);  // w term factor, :51
+    double sq[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) sq[q] = sqrt(lam[q] * lam[q] + epsl * epsl);
 #pragma unroll
     for (int sgn = 0; sgn < 2; sgn++) {
         double* f = sgn == 0 ? fp : fm;
         double lp[3];
 #pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const double sq = sqrt(lam[q] * lam[q] + epsl * epsl);
-            lp[q] = sgn == 0 ? (lam[q] + sq) / 2.0 : (lam[q] - sq) / 2.0;
-        }
-        const double wii = (3.0 - gamma) * (lp[1] + lp[2]) * (alpha * alpha) / (2.0 * (gamma - 1.0));
-        f[0] = (lp[0] * 2.0 * (gamma - 1.0) + lp[1] + lp[2]) * rho / (2.0 * gamma);
+        for (int q = 0; q < 3; q++) lp[q] = sgn == 0 ? 0.5 * (lam[q] + sq[q]) : 0.5 * (lam[q] - sq[q]);
+        const double wii = (lp[1] + lp[2]) * wc;
+        f[0] = (lp[0] * g2 + lp[1] + lp[2]) * r2g;
         if (DIR == 0) {
-            f[1] = (lp[0] * 2.0 * (gamma - 1.0) * u + lp[1] * (u - alpha) + lp[2] * (u + alpha)) * rho /
-                   (2.0 * gamma);
-            f[2] = (lp[0] * 2.0 * (gamma - 1.0) * v + lp[1] * v + lp[2] * v) * rho / (2.0 * gamma);
+            f[1] = (lp[0] * g2 * u + lp[1] * (u - alpha) + lp[2] * (u + alpha)) * r2g;
+            f[2] = (lp[0] * g2 * v + lp[1] * v + lp[2] * v) * r2g;
             f[3] = (lp[0] * (gamma - 1.0) * uu2 + lp[1] * 0.5 * ((u - alpha) * (u - alpha) + v * v) +
                     lp[2] * 0.5 * ((u + alpha) * (u + alpha) + v * v) + wii) *
-                   rho / (2.0 * gamma);
+                   r2g;
         } else {
-            f[1] = (lp[0] * 2.0 * (gamma - 1.0) * u + lp[1] * u + lp[2] * u) * rho / (2.0 * gamma);
-            f[2] = (lp[0] * 2.0 * (gamma - 1.0) * v + lp[1] * (v - alpha) + lp[2] * (v + alpha)) * rho /
-                   (2.0 * gamma);
+            f[1] = (lp[0] * g2 * u + lp[1] * u + lp[2] * u) * r2g;
+            f[2] = (lp[0] * g2 * v + lp[1] * (v - alpha) + lp[2] * (v + alpha)) * r2g;
             f[3] = (lp[0] * (gamma - 1.0) * uu2 + lp[1] * 0.5 * ((v - alpha) * (v - alpha) + u * u) +
                     lp[2] * 0.5 * ((v + alpha) * (v + alpha) + u * u) + wii) *
-                   rho / (2.0 * gamma);
+                   r2g;
         }
     }
 }

@@ -85,9 +85,9 @@ __device__ __forceinline__ void face_cw_sw(const double* __restrict__ s, int bas
 //   projection, WENO-Z, back-proj.   scheme_weno5th_CH.f90:64-81
 //   AdaWENO indicator rho+F_n, theta scheme_weno5th_SC.f90:1-64, flux_splitting_SC.f90:58-59
 //   shared-weight branch (theta>delta) scheme_weno5th_SC.f90:110-124
-// The eigenvectors are written in the frame (normal, tangential); the
-// projection sums run over the conserved components in their original order
-// (rho, rho u, rho v, E) so each sum rounds as the reference's does.
+// The eigenvectors are written in the frame (normal, tangential); the Roe
+// average and the eigenvector entries share the reciprocals of rho_L, rho_R,
+// sqrt(rho_L)+sqrt(rho_R) and the Roe sound speed.
 template <int DIR, bool ADAPT, int NC, int ST>
 __device__ __forceinline__ void face_char(const double* __restrict__ s, int base, double alpha, double gamma,
                                           double delta, double* h)
@@ -106,7 +106,7 @@ __device__ __forceinline__ void face_char(const double* __restrict__ s, int base
             const double un = s[VN * NC + o];
             lfi[i] = s[o] + 0.5 * ((un + alpha) * s[NRM * NC + o] + s[REC_P * NC + o]);
         }
-        weno5z_omega<false>(lfi, wp0, wp1, wp2, &th);
+        weno5z_omega(lfi, wp0, wp1, wp2, &th);
         chp = !(th > delta);
 #pragma unroll
         for (int i = 0; i < 5; i++) {
@@ -114,7 +114,7 @@ __device__ __forceinline__ void face_char(const double* __restrict__ s, int base
             const double un = s[VN * NC + o];
             lfi[i] = s[o] + 0.5 * ((un - alpha) * s[NRM * NC + o] + s[REC_P * NC + o]);
         }
-        weno5z_omega<false>(lfi, wm0, wm1, wm2, &th);
+        weno5z_omega(lfi, wm0, wm1, wm2, &th);
         chm = !(th > delta);
     }
 
@@ -125,21 +125,26 @@ __device__ __forceinline__ void face_char(const double* __restrict__ s, int base
         const int oL = base + 2 * ST, oR = base + 3 * ST;
         const double rhol = s[oL], ul = s[REC_U * NC + oL], vl = s[REC_V * NC + oL], pl = s[REC_P * NC + oL];
         const double rhor = s[oR], ur = s[REC_U * NC + oR], vr = s[REC_V * NC + oR], pr = s[REC_P * NC + oR];
-        const double hl = pl / (gamma - 1) / rhol + 0.5 * (ul * ul + vl * vl) + pl / rhol;
-        const double hr = pr / (gamma - 1) / rhor + 0.5 * (ur * ur + vr * vr) + pr / rhor;
+        // enthalpy H = p/(gamma-1)/rho + (u^2+v^2)/2 + p/rho = gamma/(gamma-1) p/rho + (u^2+v^2)/2
+        const double gm1 = gamma - 1.0, gg1 = gamma * rcp(gm1);
+        const double hl = gg1 * (pl * rcp(rhol)) + 0.5 * (ul * ul + vl * vl);
+        const double hr = gg1 * (pr * rcp(rhor)) + 0.5 * (ur * ur + vr * vr);
         const double sr = sqrt(rhor), sl = sqrt(rhol);
-        const double u0 = (sr * ur + sl * ul) / (sr + sl);
-        const double v0 = (sr * vr + sl * vl) / (sr + sl);
-        h0 = (sr * hr + sl * hl) / (sr + sl);
-        a0 = sqrt((gamma - 1.0) * (h0 - 0.5 * (u0 * u0 + v0 * v0)));
+        const double isum = rcp(sr + sl);
+        const double u0 = (sr * ur + sl * ul) * isum;
+        const double v0 = (sr * vr + sl * vl) * isum;
+        h0 = (sr * hr + sl * hl) * isum;
+        a0 = sqrt(gm1 * (h0 - 0.5 * (u0 * u0 + v0 * v0)));
         un0 = DIR == 0 ? u0 : v0;
         ut0 = DIR == 0 ? v0 : u0;
         uu0 = 0.5 * (u0 * u0 + v0 * v0);
-        const double hh0 = a0 * a0 / (gamma - 1.0);
-        const double A = 0.5 * uu0 / hh0, B = 0.5 * un0 / a0, Cc = 0.5 * un0 / hh0, D = 0.5 / a0;
-        const double E = 0.5 * ut0 / hh0, F = 0.5 / hh0;
+        // 1/hh0 = (gamma-1)/a0^2, with hh0 = a0^2/(gamma-1) (scheme_weno5th_SC.f90:277-328)
+        const double ia = rcp(a0);
+        const double ihh = gm1 * (ia * ia);
+        const double A = 0.5 * uu0 * ihh, B = 0.5 * un0 * ia, Cc = 0.5 * un0 * ihh, D = 0.5 * ia;
+        const double E = 0.5 * ut0 * ihh, F = 0.5 * ihh;
         Lr[0] = A + B;        Ln[0] = -Cc - D;     Lt[0] = -E;          Le[0] = F;
-        Lr[1] = -(uu0 / hh0) + 1.0;  Ln[1] = un0 / hh0;  Lt[1] = ut0 / hh0;  Le[1] = -(1.0 / hh0);
+        Lr[1] = 1.0 - uu0 * ihh;  Ln[1] = un0 * ihh;  Lt[1] = ut0 * ihh;  Le[1] = -ihh;
         Lr[2] = A - B;        Ln[2] = -Cc + D;     Lt[2] = -E;          Le[2] = F;
     }
 
@@ -185,7 +190,7 @@ __device__ __forceinline__ void face_char(const double* __restrict__ s, int base
                     for (int i = 0; i < 5; i++)
                         lf[i] = l4[0] * F[i][0] + l4[1] * F[i][1] + l4[2] * F[i][2] + l4[3] * F[i][3];
                 }
-                const double cv = weno5z<!ADAPT>(lf);
+                const double cv = weno5z(lf);
                 // back-projection h(c) = sum_k r(k,c) cv_k, accumulated in k order
                 if (kc == 0) {
                     acc_r = cv;

@@ -105,11 +105,11 @@ __global__ void __launch_bounds__(256) theta_kernel(StageArgs a, double alpha, d
     double lf[5], w0, w1, w2, th;
 #pragma unroll
     for (int i = 0; i < 5; i++) lf[i] = rho[i] + 0.5 * ((un[i] + alpha) * mn[i] + p[i]);
-    weno5z_omega<false>(lf, w0, w1, w2, &th);
+    weno5z_omega(lf, w0, w1, w2, &th);
     thp[idx] = th;
 #pragma unroll
     for (int i = 0; i < 5; i++) lf[i] = rho[5 - i] + 0.5 * ((un[5 - i] - alpha) * mn[5 - i] + p[5 - i]);
-    weno5z_omega<false>(lf, w0, w1, w2, &th);
+    weno5z_omega(lf, w0, w1, w2, &th);
     thn[idx] = th;
 }
 

@@ -36,3 +36,50 @@ def golden(name):
 def oracle():
     import pyoracle
     return pyoracle
+
+
+# ---------------------------------------------------------------- GPU parity bar
+# The HIP path evaluates the reference's formulas in FP64 with FMA contraction
+# and reciprocal-based division, so it agrees with the reference (and with the
+# oracle, its bitwise restatement) to rounding level, not bit for bit.  The
+# workloads state the bar: relative L-inf <= 1e-11 on smooth flows (100 steps)
+# and density L1 = sum |d rho| dx dy <= 1e-9 on shocked flows (20 steps).  Every
+# GPU-vs-reference comparison applies both, per conserved component:
+# max |got - ref| over the component's scale, its own largest magnitude or
+# 1e-3 of the state's largest, whichever is larger (a component that is zero
+# by symmetry, such as the Richtmyer-Meshkov transverse momentum, holds only
+# rounding noise of 1e-11 and has no scale of its own).  The measured
+# differences are 1e-16 .. 1e-13.
+# Step sizes are compared to DT_RTOL.  Within the build (tile shapes, batch vs
+# single block, graph replay vs single steps) results stay bit for bit.
+REL_LINF = 1e-11
+L1_RHO = 1e-9
+DT_RTOL = 1e-12
+
+
+def parity_errors(got, ref, dx=1.0, dy=1.0):
+    """(max per-component relative L-inf, density L1) of got vs ref, arrays (4, nj, ni)."""
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    assert got.shape == ref.shape, (got.shape, ref.shape)
+    rel = 0.0
+    floor = 1e-3 * float(np.max(np.abs(ref)))
+    for k in range(ref.shape[0]):
+        scale = max(float(np.max(np.abs(ref[k]))), floor)
+        d = float(np.max(np.abs(got[k] - ref[k])))
+        rel = max(rel, d / scale if scale > 0 else d)
+    l1 = float(np.sum(np.abs(got[0] - ref[0])) * dx * dy)
+    return rel, l1
+
+
+def assert_parity(got, ref, dx=1.0, dy=1.0, what=""):
+    assert np.all(np.isfinite(got)), f"{what}: non-finite values"
+    rel, l1 = parity_errors(got, ref, dx, dy)
+    assert rel <= REL_LINF and l1 <= L1_RHO, f"{what}: relative L-inf {rel:.3e}, density L1 {l1:.3e}"
+    return rel, l1
+
+
+def assert_dts(got, ref, what=""):
+    got, ref = np.asarray(got, dtype=np.float64), np.asarray(ref, dtype=np.float64)
+    assert got.shape == ref.shape, f"{what}: {got.shape} vs {ref.shape} steps"
+    np.testing.assert_allclose(got, ref, rtol=DT_RTOL, atol=0, err_msg=f"{what}: step sizes")

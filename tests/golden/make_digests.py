@@ -36,8 +36,10 @@ CASES = {
     "dmr_104_1920x480_t0.2": (5, 1920, 480, 104, 255, 0.6, 10 ** 6, 0.2, {}),
     # the reference's benchmark 1 (its own vortex, extrapolated x sides) at 200^2, 100 steps
     "ivc_101_253_200x200_100": (1, 200, 200, 101, 253, 0.5, 100, None, {}),
-    # Rayleigh-Taylor, AdaWENO with gravity: 1024x4096 for 20 steps, 256x1024 to t = 1.95
+    # Rayleigh-Taylor, AdaWENO with gravity: 1024x4096 for 20 steps, 256x1024 to
+    # t = 0.975 and to t = 1.95
     "rt_105_1024x4096_20": (3, 1024, 4096, 105, 255, 0.5, 20, None, RT),
+    "rt_105_256x1024_t0.975": (3, 256, 1024, 105, 255, 0.5, 10 ** 6, 0.975, RT),
     "rt_105_256x1024_t1.95": (3, 256, 1024, 105, 255, 0.5, 10 ** 6, 1.95, RT),
 }
 

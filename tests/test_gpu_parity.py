@@ -1,13 +1,16 @@
-"""HIP path vs the reference outputs and the oracle, bitwise.
+"""HIP path vs the reference outputs and the oracle.
 
-The kernels keep the reference's FP64 evaluation order and are built
-without FMA contraction, so the bar is bit-for-bit equality (stricter than
-the 1e-11 relative / 1e-9 L1 tolerances the task sets).
+The GPU evaluates the reference's formulas with FMA contraction and
+reciprocal-based division, so it is held to the workloads' tolerances
+(conftest.assert_parity: relative L-inf <= 1e-11 per conserved component and
+density L1 <= 1e-9; step sizes to 1e-12 relative) against the fixtures the
+reference solver wrote, and to bit-for-bit equality within the build (tile
+shapes, graph replay vs single steps).
 """
 import numpy as np
 import pytest
 
-from conftest import dense, golden, golden_cases
+from conftest import assert_dts, assert_parity, dense, golden, golden_cases
 
 pytestmark = pytest.mark.gpu
 CASES = golden_cases()
@@ -41,28 +44,35 @@ def test_gpu_matches_reference_golden(name, oracle):
         for _ in range(c["steps"]):
             dts.append(s.step())
         got = s.interior()
-    assert np.array_equal(got, g["cons"]), f"max |diff| = {np.max(np.abs(got - g['cons']))}"
-    assert np.array_equal(np.array(dts), g["dts"])
-    assert s.current_time == float(g["time"])
+        dx, dy = s.cfg.dx, s.cfg.dy
+    rel, l1 = assert_parity(got, g["cons"], dx, dy, name)
+    print(f"{name}: relative L-inf {rel:.2e}, density L1 {l1:.2e}", flush=True)
+    assert_dts(dts, g["dts"], name)
+    assert s.current_time == pytest.approx(float(g["time"]), rel=1e-12, abs=0)
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("name", ["dmr_104", "ivc_101_253", "rt_105", "sv_101_255", "ivc_101_251", "ivc_102_251",
                                   "dmr_103_251", "ivc_103_255"])
-def test_gpu_tile_shape_invariance(name, tile, oracle):
+def test_gpu_tile_shape_invariance(name, oracle):
     """Every tile shape gives the same bits (faces are independent; the only
-    cross-tile reduction is a max).  The upwind5 cases cover the 512-thread
-    tiles whose face fluxes overwrite the cell records in LDS; the
-    Steger-Warming cases the split-flux bands of every shape."""
+    cross-tile reduction is a max), and the reference's state within the
+    tolerances.  The upwind5 cases cover the 512-thread tiles whose face
+    fluxes overwrite the cell records in LDS; the Steger-Warming cases the
+    split-flux bands of every shape."""
     c = CASES[name]
     g = golden(name)
     o = _oracle_ic(oracle, c)
-    print(f"case {name} tile {tile}", flush=True)
-    with _solver(c, tile=tile) as s:
-        s.set_state(o.cons())
-        s.step(c["steps"])
-        got = s.interior()
-    assert np.array_equal(got, g["cons"])
+    out = {}
+    for tile in (0, 1, 2, 3, 4, 5):
+        print(f"case {name} tile {tile}", flush=True)
+        with _solver(c, tile=tile) as s:
+            s.set_state(o.cons())
+            s.step(c["steps"])
+            out[tile] = s.interior()
+            dx, dy = s.cfg.dx, s.cfg.dy
+    assert_parity(out[0], g["cons"], dx, dy, name)
+    for tile in (1, 2, 3, 4, 5):
+        assert np.array_equal(out[tile], out[0]), f"tile {tile}"
 
 
 @pytest.mark.parametrize("name", ["dmr_104", "ssl_104"])
@@ -91,8 +101,8 @@ def test_gpu_run_stops_at_total_time(oracle, name, total):
         s.set_state(_oracle_ic(oracle, c).cons())
         s.run(1000, total_time=total)
         assert s.current_step == o.current_step
-        assert s.current_time == o.time
-        assert np.array_equal(s.interior(), o.interior())
+        assert s.current_time == pytest.approx(o.time, rel=1e-12, abs=0) and s.current_time >= total
+        assert_parity(s.interior(), o.interior(), s.cfg.dx, s.cfg.dy, name)
 
 
 @pytest.mark.parametrize("sp,rc", [(101, 251), (102, 251), (103, 255), (104, 255), (105, 255)])

@@ -5,9 +5,11 @@ blown up (DENSE_ERR_NONFINITE).
 The expected values are the reference's formulas evaluated with numpy in the
 reference's order (get_primary.f90:46-73): u = m/rho, p = (gamma-1)(E -
 0.5 rho (u^2 + v^2)) with the single-precision floor, c = sqrt(gamma p / rho),
-max|u|+c and max|v|+c over the interior, s = ln p - ln rho.  numpy's sqrt and
-division are correctly rounded, like the device's, and the entropy's log is
-the C library's on both sides, so the comparison is exact.
+max|u|+c and max|v|+c over the interior, s = ln p - ln rho.  The device divides
+through a refined reciprocal of rho and contracts multiply-adds, so the
+primitives agree to a few ulps (checked to 1e-14 relative); the entropy is
+formed on the host from the returned p and rho with the C library's log, so
+that comparison is exact.
 """
 import math
 
@@ -48,7 +50,8 @@ def test_max_eig_is_the_current_state_wave_speed():
             cons = s.interior()
             _, u, v, _, c = _prims(cons, 1.4)
             got = s.max_eig()
-            assert got == (float(np.max(np.abs(u) + c)), float(np.max(np.abs(v) + c)))
+            want = (float(np.max(np.abs(u) + c)), float(np.max(np.abs(v) + c)))
+            assert got == pytest.approx(want, rel=1e-14, abs=0)
             s.step(2)
 
 
@@ -61,12 +64,13 @@ def test_get_state_primitives_ghosts_and_entropy():
         inner = (slice(None), slice(BF, BF + n), slice(BF, BF + n))
         rho, u, v, p, c = _prims(cons[inner], 1.4)
         for k, q in enumerate((rho, u, v, p, c)):
-            assert np.array_equal(prim[k][inner[1:]], q), f"primitive {k}"
+            got = prim[k][inner[1:]]
+            assert np.max(np.abs(got - q)) <= 1e-14 * np.max(np.abs(q)), f"primitive {k}"
         # the entropy is formed on the host with the C library's log, as the
         # reference's Fortran runtime forms it (numpy's vectorised log may
         # round differently): compare with math.log
         glibc_log = np.vectorize(math.log)
-        assert np.array_equal(prim[5][inner[1:]], glibc_log(p) - glibc_log(rho))
+        assert np.array_equal(prim[5][inner[1:]], glibc_log(prim[3][inner[1:]]) - glibc_log(prim[0][inner[1:]]))
         # periodic ghost layers: conserved and primitive copies of the far side
         for arr in (cons, prim[:5]):
             assert np.array_equal(arr[:, BF:BF + n, :BF], arr[:, BF:BF + n, n:n + BF])
