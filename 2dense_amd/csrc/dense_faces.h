@@ -5,11 +5,19 @@
 #pragma once
 #include "dense_device.h"
 
-#ifndef DENSE_SIGN_SERIAL
-#define DENSE_SIGN_SERIAL 0
-#endif
-
 namespace dense {
+
+// Characteristic WENO-Z (switch 104) reconstructs its two flux signs one after
+// the other, at 3 waves per SIMD (167 VGPRs, 76 B/lane of spill): 82 us per
+// stage launch on double Mach 1920x480 against 90 us with the signs
+// interleaved at 2 waves (243 VGPRs).  AdaWENO (105), which holds its
+// indicator weights across both signs, stays interleaved at 2 waves (serial at
+// 3 waves measured 7 % slower on double Mach, 3 % on Rayleigh-Taylor).
+template <bool ADAPT>
+constexpr bool sign_serial()
+{
+    return !ADAPT;
+}
 
 // LDS record slots of one cell: 0..3 conserved (rho, rho u, rho v, E), then u, v, p, c.
 enum { REC_U = 4, REC_V = 5, REC_P = 6, REC_C = 7, REC_N = 8 };
@@ -148,13 +156,11 @@ __device__ __forceinline__ void face_char(const double* __restrict__ s, int base
         Lr[2] = A - B;        Ln[2] = -Cc + D;     Lt[2] = -E;          Le[2] = F;
     }
 
-    // DENSE_SIGN_SERIAL=1 runs the two signs one after the other (fewer live
-    // registers); 0 lets the scheduler interleave them (more ILP per wave).
-#if DENSE_SIGN_SERIAL
-#pragma unroll 1
-#else
-#pragma unroll
-#endif
+    // One sign per iteration: h[0..3] for F+, h[4..7] for F-.  Serial signs
+    // (fewer live registers) or both unrolled and interleaved by the scheduler
+    // (more ILP per wave), see sign_serial.
+    constexpr int SIGN_UNROLL = sign_serial<ADAPT>() ? 1 : 2;
+#pragma unroll SIGN_UNROLL
     for (int sg = 0; sg < 2; sg++) {
         const bool plus = sg == 0;
         const bool ch = plus ? chp : chm;

@@ -188,9 +188,15 @@ constexpr bool faces_read_sound_speed()
     return SCHEME <= S_LLF_Z || (SCHEME >= S_SW_UP5 && SCHEME <= S_SW_Z);
 }
 
-// Minimum waves per SIMD requested from the register allocator, per scheme.
-#ifndef DENSE_WAVES_CHAR
-#define DENSE_WAVES_CHAR 2
+// Minimum waves per SIMD requested from the register allocator, per scheme:
+// 3 for characteristic WENO-Z (its signs run one after the other, see
+// sign_serial), 2 for AdaWENO, 3 for the component-wise schemes (4 for the
+// aliased upwind5 tiles below).
+#ifndef DENSE_WAVES_CH
+#define DENSE_WAVES_CH 3
+#endif
+#ifndef DENSE_WAVES_SC
+#define DENSE_WAVES_SC 2
 #endif
 #ifndef DENSE_WAVES_CW
 #define DENSE_WAVES_CW 3
@@ -215,9 +221,10 @@ constexpr bool alias_h()
 template <int SCHEME, class T>
 constexpr int min_waves()
 {
-    return (SCHEME == S_CH || SCHEME == S_SC) ? DENSE_WAVES_CHAR
-           : alias_h<SCHEME, T>()             ? 4
-                                              : DENSE_WAVES_CW;
+    return SCHEME == S_CH              ? DENSE_WAVES_CH
+           : SCHEME == S_SC            ? DENSE_WAVES_SC
+           : alias_h<SCHEME, T>()      ? 4
+                                       : DENSE_WAVES_CW;
 }
 
 // Per-XCD tile queue of a persistent launch: queue x owns the contiguous
