@@ -99,11 +99,14 @@ def load_library(path: str | None = None) -> C.CDLL:
     lib.dense_autotune.argtypes = [H, C.c_int, C.POINTER(C.c_int), dp]
     lib.dense_get_theta.argtypes = [H, dp]
     lib.dense_sync.argtypes = [H]
+    lib.dense_probe_fp64.argtypes = [dp]
+    lib.dense_probe_hbm.argtypes = [C.c_size_t, dp, dp]
     lib.dense_last_error.argtypes = [H]
     lib.dense_last_error.restype = C.c_char_p
     for name in ("dense_problem_defaults", "dense_create", "dense_destroy", "dense_set_state",
                  "dense_get_state", "dense_get_dump", "dense_step", "dense_run", "dense_run_saving", "dense_max_eig",
-                 "dense_last_run_gpu_ms", "dense_update_params", "dense_autotune", "dense_get_theta", "dense_sync"):
+                 "dense_last_run_gpu_ms", "dense_update_params", "dense_autotune", "dense_get_theta", "dense_sync",
+                 "dense_probe_fp64", "dense_probe_hbm"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -307,6 +310,17 @@ class Solver:
 
     def sync(self):
         _check(self.lib, self.h, self.lib.dense_sync(self.h))
+
+
+def probe_peaks(hbm_bytes: int = 2 << 30) -> dict:
+    """Roofline calibration of the current device: FP64 FMA TFLOP/s and the
+    HBM streaming read and copy rates in GB/s (buffers of hbm_bytes)."""
+    lib = load_library()
+    t = C.c_double(0.0)
+    r, c = C.c_double(0.0), C.c_double(0.0)
+    _check(lib, None, lib.dense_probe_fp64(C.byref(t)))
+    _check(lib, None, lib.dense_probe_hbm(hbm_bytes, C.byref(r), C.byref(c)))
+    return dict(fp64_tflops=t.value, hbm_read_gbs=r.value, hbm_copy_gbs=c.value, hbm_bytes=hbm_bytes)
 
 
 def cell_updates(ni: int, nj: int, steps: int) -> int:

@@ -27,7 +27,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FC", "/opt/rocm/lib/llvm/bin/flang")
 ARCH = os.environ.get("DENSE_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["dense_kernels.hip", "dense_batch.hip", "dense_capi.hip", "dense_adjoint.hip"]
+HIP_SOURCES = ["dense_kernels.hip", "dense_batch.hip", "dense_capi.hip", "dense_adjoint.hip", "dense_probe.hip"]
 HIP_FLAGS = [
     "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
     "-ffp-contract=on", "-fno-fast-math",

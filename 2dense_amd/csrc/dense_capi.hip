@@ -33,8 +33,8 @@ struct dense_solver {
     double* stage_host = nullptr; // pinned interior staging for set_state
     hipGraphExec_t graph = nullptr;      // GRAPH_CHUNK steps
     int graph_steps = 0;
-    hipGraphExec_t graph_rem = nullptr;  // the remainder max_steps % GRAPH_CHUNK of a run
-    int graph_rem_steps = 0;
+    hipGraphExec_t graph_rem[16] = {};   // [r]: r < GRAPH_CHUNK steps, the remainder of a run
+    int graph_rem_steps[16] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;  // bracket the last dense_run on the stream
     double last_run_ms = 0.0;
     int last_run_launches = 0;
@@ -232,9 +232,11 @@ static void release_graph(dense_solver* h)
     if (h->graph) (void)hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
     h->graph_steps = 0;
-    if (h->graph_rem) (void)hipGraphExecDestroy(h->graph_rem);
-    h->graph_rem = nullptr;
-    h->graph_rem_steps = 0;
+    for (int r = 0; r < 16; r++) {
+        if (h->graph_rem[r]) (void)hipGraphExecDestroy(h->graph_rem[r]);
+        h->graph_rem[r] = nullptr;
+        h->graph_rem_steps[r] = 0;
+    }
 }
 
 extern "C" int dense_create(const dense_cfg* cfg, dense_handle* out)
@@ -495,6 +497,7 @@ static int ensure_hist(dense_solver* h, int n)
 }
 
 constexpr int GRAPH_CHUNK = 16;
+static_assert(GRAPH_CHUNK <= 16, "one cached remainder graph per count below the chunk");
 
 // The loop general_main.f90:23-40 on the device: up to max_steps steps while
 // t < total_time, step < total_step and !(elasp >= dt_save), the clock, step
@@ -547,8 +550,8 @@ static int run_steps(dense_solver* h, int max_steps, double total_time, int tota
         int rc = build_graph(h, GRAPH_CHUNK, &h->graph, &h->graph_steps);
         if (rc) return rc;
     }
-    if (rem > 0 && h->graph_rem_steps != rem) {
-        int rc = build_graph(h, rem, &h->graph_rem, &h->graph_rem_steps);
+    if (rem > 0 && h->graph_rem_steps[rem] != rem) {
+        int rc = build_graph(h, rem, &h->graph_rem[rem], &h->graph_rem_steps[rem]);
         if (rc) return rc;
     }
     // the device stops itself once the clock passes total_time/total_step/dt_save
@@ -557,7 +560,7 @@ static int run_steps(dense_solver* h, int max_steps, double total_time, int tota
     HIPCHK(h, launch_set_clock(h->st, *t, total_time, *step, step_end, h->stream, 0.0, 0.0, el, dt_save));
     HIPCHK(h, hipEventRecord(h->ev0, h->stream));
     for (int i = 0; i < full; i++) HIPCHK(h, hipGraphLaunch(h->graph, h->stream));
-    if (rem > 0) HIPCHK(h, hipGraphLaunch(h->graph_rem, h->stream));
+    if (rem > 0) HIPCHK(h, hipGraphLaunch(h->graph_rem[rem], h->stream));
     HIPCHK(h, hipEventRecord(h->ev1, h->stream));
     int rc = read_state(h);
     if (rc) return rc;

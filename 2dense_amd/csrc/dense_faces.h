@@ -87,7 +87,7 @@ __device__ __forceinline__ void face_cw_sw(const double* __restrict__ s, int bas
     }
 }
 
-// Characteristic WENO-Z (switch 104) and AdaWENO (switch 105) at one face.
+// ---- pieces of the characteristic faces (switch 104 and 105)
 //   Roe average of cells f-1, f      scheme_weno5th_CH.f90:44-62
 //   eigenvectors L, R                scheme_weno5th_SC.f90:277-328 (x), :607-658 (y)
 //   projection, WENO-Z, back-proj.   scheme_weno5th_CH.f90:64-81
@@ -96,65 +96,164 @@ __device__ __forceinline__ void face_cw_sw(const double* __restrict__ s, int bas
 // The eigenvectors are written in the frame (normal, tangential); the Roe
 // average and the eigenvector entries share the reciprocals of rho_L, rho_R,
 // sqrt(rho_L)+sqrt(rho_R) and the Roe sound speed.
+
+// Roe state and left eigenvectors of one face.  Chars 1, 2 and 4 have entries
+// for (rho, n, t, E) in Lr/Ln/Lt/Le[0..2]; char 3 is (-u_t, [0|1], [1|0], 0).
+struct RoeFace {
+    double Lr[3], Ln[3], Lt[3], Le[3];
+    double ut0, un0, a0, h0, uu0;
+};
+
+template <int DIR, int NC, int ST>
+__device__ __forceinline__ void roe_face(const double* __restrict__ s, int base, double gamma, RoeFace& R)
+{
+    const int oL = base + 2 * ST, oR = base + 3 * ST;
+    const double rhol = s[oL], ul = s[REC_U * NC + oL], vl = s[REC_V * NC + oL], pl = s[REC_P * NC + oL];
+    const double rhor = s[oR], ur = s[REC_U * NC + oR], vr = s[REC_V * NC + oR], pr = s[REC_P * NC + oR];
+    // enthalpy H = p/(gamma-1)/rho + (u^2+v^2)/2 + p/rho = gamma/(gamma-1) p/rho + (u^2+v^2)/2
+    const double gm1 = gamma - 1.0, gg1 = gamma * rcp(gm1);
+    const double hl = gg1 * (pl * rcp(rhol)) + 0.5 * (ul * ul + vl * vl);
+    const double hr = gg1 * (pr * rcp(rhor)) + 0.5 * (ur * ur + vr * vr);
+    const double sr = sqrt(rhor), sl = sqrt(rhol);
+    const double isum = rcp(sr + sl);
+    const double u0 = (sr * ur + sl * ul) * isum;
+    const double v0 = (sr * vr + sl * vl) * isum;
+    R.h0 = (sr * hr + sl * hl) * isum;
+    R.a0 = sqrt(gm1 * (R.h0 - 0.5 * (u0 * u0 + v0 * v0)));
+    R.un0 = DIR == 0 ? u0 : v0;
+    R.ut0 = DIR == 0 ? v0 : u0;
+    R.uu0 = 0.5 * (u0 * u0 + v0 * v0);
+    // 1/hh0 = (gamma-1)/a0^2, with hh0 = a0^2/(gamma-1) (scheme_weno5th_SC.f90:277-328)
+    const double ia = rcp(R.a0);
+    const double ihh = gm1 * (ia * ia);
+    const double A = 0.5 * R.uu0 * ihh, B = 0.5 * R.un0 * ia, Cc = 0.5 * R.un0 * ihh, D = 0.5 * ia;
+    const double E = 0.5 * R.ut0 * ihh, F = 0.5 * ihh;
+    R.Lr[0] = A + B;             R.Ln[0] = -Cc - D;         R.Lt[0] = -E;              R.Le[0] = F;
+    R.Lr[1] = 1.0 - R.uu0 * ihh;  R.Ln[1] = R.un0 * ihh;    R.Lt[1] = R.ut0 * ihh;     R.Le[1] = -ihh;
+    R.Lr[2] = A - B;             R.Ln[2] = -Cc + D;         R.Lt[2] = -E;              R.Le[2] = F;
+}
+
+// The 5-point split-flux stencil of one sign: + side F+(f-3+i), - side
+// F-(f+2-i), i = 0..4 (scheme_weno5z_general.f90:81, :106 with GLF fluxes).
+template <int DIR, int NC, int ST>
+__device__ __forceinline__ void flux_stencil(const double* __restrict__ s, int base, bool plus, double alpha,
+                                             double F[5][4])
+{
+    constexpr int VN = DIR == 0 ? REC_U : REC_V;
+    const double sa = plus ? alpha : -alpha;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        const int o = base + (plus ? i : 5 - i) * ST;
+        const double un = s[VN * NC + o];
+        const double upa = un + sa;
+        const double c4[4] = {s[o], s[NC + o], s[2 * NC + o], s[3 * NC + o]};
+        lf_flux<DIR>(upa, un, s[REC_P * NC + o], c4, F[i]);
+    }
+}
+
+// Characteristic reconstruction of one sign: project the stencil on the left
+// eigenvectors, WENO-Z per characteristic field, back-project with R.
+template <int DIR>
+__device__ __forceinline__ void char_recon(const double F[5][4], const RoeFace& R, double* r4)
+{
+    constexpr int NRM = DIR == 0 ? 1 : 2, TAN = 3 - NRM;
+    double acc_r = 0, acc_n = 0, acc_t = 0, acc_e = 0;
+#pragma unroll
+    for (int kc = 0; kc < 4; kc++) {
+        double lf[5];
+        if (kc == 2) {
+            // l(:,3) = (-u_t, [0|1], [1|0], 0): (-u_t)*F_rho + F_t
+#pragma unroll
+            for (int i = 0; i < 5; i++) lf[i] = (-R.ut0) * F[i][0] + F[i][TAN];
+        } else {
+            const int q = kc == 3 ? 2 : kc;
+            double l4[4];
+            l4[0] = R.Lr[q];
+            l4[NRM] = R.Ln[q];
+            l4[TAN] = R.Lt[q];
+            l4[3] = R.Le[q];
+#pragma unroll
+            for (int i = 0; i < 5; i++)
+                lf[i] = l4[0] * F[i][0] + l4[1] * F[i][1] + l4[2] * F[i][2] + l4[3] * F[i][3];
+        }
+        const double cv = weno5z(lf);
+        // back-projection h(c) = sum_k r(k,c) cv_k, accumulated in k order
+        if (kc == 0) {
+            acc_r = cv;
+            acc_n = (R.un0 - R.a0) * cv;
+            acc_t = R.ut0 * cv;
+            acc_e = (R.h0 - R.a0 * R.un0) * cv;
+        } else if (kc == 1) {
+            acc_r = acc_r + cv;
+            acc_n = acc_n + R.un0 * cv;
+            acc_t = acc_t + R.ut0 * cv;
+            acc_e = acc_e + R.uu0 * cv;
+        } else if (kc == 2) {
+            acc_t = acc_t + cv;
+            acc_e = acc_e + R.ut0 * cv;
+        } else {
+            acc_r = acc_r + cv;
+            acc_n = acc_n + (R.un0 + R.a0) * cv;
+            acc_t = acc_t + R.ut0 * cv;
+            acc_e = acc_e + (R.h0 + R.a0 * R.un0) * cv;
+        }
+    }
+    r4[0] = acc_r;
+    r4[NRM] = acc_n;
+    r4[TAN] = acc_t;
+    r4[3] = acc_e;
+}
+
+// AdaWENO's shared-weight branch of one sign: every component reconstructed
+// with the indicator's WENO-Z weights (scheme_weno5th_SC.f90:111-124).
+__device__ __forceinline__ void cw_shared(const double F[5][4], double w0, double w1, double w2, double* r4)
+{
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        double lf[5], q0, q1, q2;
+#pragma unroll
+        for (int i = 0; i < 5; i++) lf[i] = F[i][k];
+        candidates(lf, q0, q1, q2);
+        r4[k] = w0 * q0 + w1 * q1 + w2 * q2;
+    }
+}
+
+// AdaWENO's indicator of one sign: WENO-Z weights of rho + F_n on the sign's
+// stencil, and the branch: characteristic unless theta > delta (strict).
+template <int DIR, int NC, int ST>
+__device__ __forceinline__ bool adaweno_sign(const double* __restrict__ s, int base, bool plus, double alpha,
+                                            double delta, double& w0, double& w1, double& w2)
+{
+    constexpr int NRM = DIR == 0 ? 1 : 2;
+    constexpr int VN = DIR == 0 ? REC_U : REC_V;
+    double lfi[5], th;
+    const double sa = plus ? alpha : -alpha;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        const int o = base + (plus ? i : 5 - i) * ST;
+        const double un = s[VN * NC + o];
+        lfi[i] = s[o] + 0.5 * ((un + sa) * s[NRM * NC + o] + s[REC_P * NC + o]);
+    }
+    weno5z_omega(lfi, w0, w1, w2, &th);
+    return !(th > delta);
+}
+
+// Characteristic WENO-Z (switch 104) and AdaWENO (switch 105) at one face:
+// h(face) = h+ + h- into h[0..3] (h[4..7] scratch).
 template <int DIR, bool ADAPT, int NC, int ST>
 __device__ __forceinline__ void face_char(const double* __restrict__ s, int base, double alpha, double gamma,
                                           double delta, double* h)
 {
-    constexpr int NRM = DIR == 0 ? 1 : 2, TAN = 3 - NRM;
-    constexpr int VN = DIR == 0 ? REC_U : REC_V;
-
     // --- AdaWENO indicator weights per sign
     bool chp = true, chm = true;
     double wp0 = 0, wp1 = 0, wp2 = 0, wm0 = 0, wm1 = 0, wm2 = 0;
     if constexpr (ADAPT) {
-        double lfi[5], th;
-#pragma unroll
-        for (int i = 0; i < 5; i++) {
-            const int o = base + i * ST;
-            const double un = s[VN * NC + o];
-            lfi[i] = s[o] + 0.5 * ((un + alpha) * s[NRM * NC + o] + s[REC_P * NC + o]);
-        }
-        weno5z_omega(lfi, wp0, wp1, wp2, &th);
-        chp = !(th > delta);
-#pragma unroll
-        for (int i = 0; i < 5; i++) {
-            const int o = base + (5 - i) * ST;
-            const double un = s[VN * NC + o];
-            lfi[i] = s[o] + 0.5 * ((un - alpha) * s[NRM * NC + o] + s[REC_P * NC + o]);
-        }
-        weno5z_omega(lfi, wm0, wm1, wm2, &th);
-        chm = !(th > delta);
+        chp = adaweno_sign<DIR, NC, ST>(s, base, true, alpha, delta, wp0, wp1, wp2);
+        chm = adaweno_sign<DIR, NC, ST>(s, base, false, alpha, delta, wm0, wm1, wm2);
     }
-
     // --- Roe state and eigenvectors (only when a sign needs them)
-    double Lr[3], Ln[3], Lt[3], Le[3];  // chars 1,2,4: entries for rho, n, t, E
-    double ut0 = 0, un0 = 0, a0 = 0, h0 = 0, uu0 = 0;
-    if (chp || chm) {
-        const int oL = base + 2 * ST, oR = base + 3 * ST;
-        const double rhol = s[oL], ul = s[REC_U * NC + oL], vl = s[REC_V * NC + oL], pl = s[REC_P * NC + oL];
-        const double rhor = s[oR], ur = s[REC_U * NC + oR], vr = s[REC_V * NC + oR], pr = s[REC_P * NC + oR];
-        // enthalpy H = p/(gamma-1)/rho + (u^2+v^2)/2 + p/rho = gamma/(gamma-1) p/rho + (u^2+v^2)/2
-        const double gm1 = gamma - 1.0, gg1 = gamma * rcp(gm1);
-        const double hl = gg1 * (pl * rcp(rhol)) + 0.5 * (ul * ul + vl * vl);
-        const double hr = gg1 * (pr * rcp(rhor)) + 0.5 * (ur * ur + vr * vr);
-        const double sr = sqrt(rhor), sl = sqrt(rhol);
-        const double isum = rcp(sr + sl);
-        const double u0 = (sr * ur + sl * ul) * isum;
-        const double v0 = (sr * vr + sl * vl) * isum;
-        h0 = (sr * hr + sl * hl) * isum;
-        a0 = sqrt(gm1 * (h0 - 0.5 * (u0 * u0 + v0 * v0)));
-        un0 = DIR == 0 ? u0 : v0;
-        ut0 = DIR == 0 ? v0 : u0;
-        uu0 = 0.5 * (u0 * u0 + v0 * v0);
-        // 1/hh0 = (gamma-1)/a0^2, with hh0 = a0^2/(gamma-1) (scheme_weno5th_SC.f90:277-328)
-        const double ia = rcp(a0);
-        const double ihh = gm1 * (ia * ia);
-        const double A = 0.5 * uu0 * ihh, B = 0.5 * un0 * ia, Cc = 0.5 * un0 * ihh, D = 0.5 * ia;
-        const double E = 0.5 * ut0 * ihh, F = 0.5 * ihh;
-        Lr[0] = A + B;        Ln[0] = -Cc - D;     Lt[0] = -E;          Le[0] = F;
-        Lr[1] = 1.0 - uu0 * ihh;  Ln[1] = un0 * ihh;  Lt[1] = ut0 * ihh;  Le[1] = -ihh;
-        Lr[2] = A - B;        Ln[2] = -Cc + D;     Lt[2] = -E;          Le[2] = F;
-    }
+    RoeFace R;
+    if (chp || chm) roe_face<DIR, NC, ST>(s, base, gamma, R);
 
     // One sign per iteration: h[0..3] for F+, h[4..7] for F-.  Serial signs
     // (fewer live registers) or both unrolled and interleaved by the scheduler
@@ -163,76 +262,11 @@ __device__ __forceinline__ void face_char(const double* __restrict__ s, int base
 #pragma unroll SIGN_UNROLL
     for (int sg = 0; sg < 2; sg++) {
         const bool plus = sg == 0;
-        const bool ch = plus ? chp : chm;
-        const double sa = plus ? alpha : -alpha;
-        // 5-point flux stencil of this sign
         double F[5][4];
-#pragma unroll
-        for (int i = 0; i < 5; i++) {
-            const int o = base + (plus ? i : 5 - i) * ST;
-            const double un = s[VN * NC + o];
-            const double upa = un + sa;
-            const double c4[4] = {s[o], s[NC + o], s[2 * NC + o], s[3 * NC + o]};
-            lf_flux<DIR>(upa, un, s[REC_P * NC + o], c4, F[i]);
-        }
+        flux_stencil<DIR, NC, ST>(s, base, plus, alpha, F);
         double r4[4];
-        if (ch) {
-            double acc_r = 0, acc_n = 0, acc_t = 0, acc_e = 0;
-#pragma unroll
-            for (int kc = 0; kc < 4; kc++) {
-                double lf[5];
-                if (kc == 2) {
-                    // l(:,3) = (-u_t, [0|1], [1|0], 0): (-u_t)*F_rho + F_t
-#pragma unroll
-                    for (int i = 0; i < 5; i++) lf[i] = (-ut0) * F[i][0] + F[i][TAN];
-                } else {
-                    const int q = kc == 3 ? 2 : kc;
-                    double l4[4];
-                    l4[0] = Lr[q];
-                    l4[NRM] = Ln[q];
-                    l4[TAN] = Lt[q];
-                    l4[3] = Le[q];
-#pragma unroll
-                    for (int i = 0; i < 5; i++)
-                        lf[i] = l4[0] * F[i][0] + l4[1] * F[i][1] + l4[2] * F[i][2] + l4[3] * F[i][3];
-                }
-                const double cv = weno5z(lf);
-                // back-projection h(c) = sum_k r(k,c) cv_k, accumulated in k order
-                if (kc == 0) {
-                    acc_r = cv;
-                    acc_n = (un0 - a0) * cv;
-                    acc_t = ut0 * cv;
-                    acc_e = (h0 - a0 * un0) * cv;
-                } else if (kc == 1) {
-                    acc_r = acc_r + cv;
-                    acc_n = acc_n + un0 * cv;
-                    acc_t = acc_t + ut0 * cv;
-                    acc_e = acc_e + uu0 * cv;
-                } else if (kc == 2) {
-                    acc_t = acc_t + cv;
-                    acc_e = acc_e + ut0 * cv;
-                } else {
-                    acc_r = acc_r + cv;
-                    acc_n = acc_n + (un0 + a0) * cv;
-                    acc_t = acc_t + ut0 * cv;
-                    acc_e = acc_e + (h0 + a0 * un0) * cv;
-                }
-            }
-            r4[0] = acc_r;
-            r4[NRM] = acc_n;
-            r4[TAN] = acc_t;
-            r4[3] = acc_e;
-        } else {
-            const double w0 = plus ? wp0 : wm0, w1 = plus ? wp1 : wm1, w2 = plus ? wp2 : wm2;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                double lf[5], q0, q1, q2;
-#pragma unroll
-                for (int i = 0; i < 5; i++) lf[i] = F[i][k];
-                candidates(lf, q0, q1, q2);
-                r4[k] = w0 * q0 + w1 * q1 + w2 * q2;
-            }
-        }
+        if (plus ? chp : chm) char_recon<DIR>(F, R, r4);
+        else cw_shared(F, plus ? wp0 : wm0, plus ? wp1 : wm1, plus ? wp2 : wm2, r4);
         if (plus) {
 #pragma unroll
             for (int k = 0; k < 4; k++) h[k] = r4[k];

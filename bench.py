@@ -200,6 +200,8 @@ def gpu_leg(steps: int, warmup: int):
     s.set_state(dmr_initial_state(NI, NJ))
     if warmup:
         s.run(warmup)
+    if steps % 16:
+        s.run(steps % 16)  # captures the remainder graph of the timed run outside the timed region
     s.sync()
     t0 = time.perf_counter()
     taken = s.run(steps)
@@ -320,7 +322,177 @@ def adjoint_leg(n: int = 256, steps: int = 200, k: int = 20, n_modes: int = 64, 
                               worst_rel=float(np.max(np.abs(g[:fd_modes] - fd) / np.abs(fd)))))
 
 
+def probe_leg():
+    """FP64 FMA rate and HBM streaming rates measured on this device
+    (dense_probe_fp64 / dense_probe_hbm over 2 GiB, beyond the last-level cache)."""
+    D = importlib.import_module("2dense_amd")
+    return D.probe_peaks(2 << 30)
+
+
+def driver_leg(steps: int = 160):
+    """The Fortran drop-in driver (2dense_amd/bin/dense_gpu, the reference's
+    input.in and loop, general_main.f90:23-38) on the headline workload, timed
+    by its own CpuTime.dat as the reference times its loop.  It replays the
+    steps between saves from captured graphs (dense_run_saving)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    drv = os.path.join(ROOT, "2dense_amd", "bin", "dense_gpu")
+    if not os.path.exists(drv):
+        return dict(error="Fortran driver not built")
+    wd = tempfile.mkdtemp(prefix="dense_drv_")
+    try:
+        with open(os.path.join(wd, "input.in"), "w") as f:
+            f.write(_input_in(benchmark=5, ni=NI, nj=NJ, splitting=SPLITTING, recon=RECON, cfl=CFL,
+                              total_step=steps, save_step=10 ** 8, threads=1))
+        env = dict(os.environ, DENSE_AUTOTUNE="0")
+        out = subprocess.run([drv], cwd=wd, env=env, capture_output=True, text=True, timeout=600)
+        if out.returncode != 0:
+            return dict(error=out.stdout[-500:] + out.stderr[-500:])
+        with open(os.path.join(wd, "CpuTime.dat")) as f:
+            line = f.readline()
+        t = float(line.split(":")[1].split()[0].replace("D", "E"))
+        return dict(workload=f"double Mach {NI}x{NJ}, switch {SPLITTING}, {steps} steps through the Fortran driver",
+                    steps=steps, loop_s=t, cell_updates_per_s=NI * NJ * steps / t)
+    finally:
+        shutil.rmtree(wd, ignore_errors=True)
+
+
+def _input_in(*, benchmark, ni, nj, splitting, recon, cfl, total_step, save_step, threads, gamma=1.4):
+    """The reference's six NAMELIST groups (flow_mod_general.f90:313-332) with
+    output off: no dumps, no Tecplot saves before the end of the run."""
+    return f"""&OMP_PARS
+ OMP_THREAD_NUM={threads},
+ /
+&MODEL_PARS
+ NEQ=4, IF_SOURCE=F, IF_VISCOUS=F, IF_BODYFORCE=F, MACH_INF=1.0, REYNOLDS=500.0,
+ BENCHMARK_INDEX={benchmark},
+ /
+&GRID_PARS
+ IF_CURVE_GRID=F, IF_READ_FROM_FILE=F, SINGLE_NI={ni}, SINGLE_NJ={nj},
+ /
+&TIME_STEP_PARS
+ STAGES=3, TOTAL_TIME=1.0e30, DT_SAVE=1.0e30, DT=0.01, CFL={cfl!r},
+ TOTAL_STEP={total_step}, SAVE_STEP={save_step},
+ IF_CONST_DT=F, IF_SAVE_BINARY=F, IF_SAVE_TEC=F,
+ /
+&SCHEME_PARS
+ SCHEME_SPLITTING_INDEX={splitting}, SCHEME_INV_INDEX={recon}, SCHEME_VIS_INDEX=301,
+ IF_SAVE_THETA=F, DELTA=0.5,
+ /
+&GAS_PARS
+ GAMMA={gamma!r},
+ /
+"""
+
+
+# AdaWENO's algorithmic flops per cell-update with every (face, sign) pair on
+# the component-wise branch (delta = 0) and on the characteristic branch
+# (delta = 1), from the oracle's op count: python oracle/count_ops.py <bm> <ni> <nj> 105 255 <delta>.
+# The reference's cost is linear in the fraction of characteristic pairs in between.
+ADAWENO_FLOPS = {"dmr": (2013.859375, 6808.09375),            # 5 1920 480 105 255 {0,1}
+                 "rt": (2035.80908203125, 6826.7314453125)}    # 3 1024 4096 105 255 {0,1}
+
+
+def _branch_stats(th, delta, ni, nj, tx=16, ty=15, nt=256):
+    """Fraction of (face, sign) pairs on the characteristic branch (theta <=
+    delta) and the fraction of wavefronts, per direction and sign, whose 64
+    faces take both branches, for the automatic 16x15 tile with 256 threads
+    (x faces: thread t -> face (t % 17, t // 17); y faces: (t % 16, t // 16))."""
+    xp, xn, ep, en = th
+    nchar = sum(int(np.count_nonzero(a <= delta)) for a in th)
+    npairs = sum(a.size for a in th)
+    tiles_x, tiles_y = -(-ni // tx), -(-nj // ty)
+    t = np.arange(nt)
+    X0 = (np.arange(tiles_x) * tx)[None, :, None]
+    Y0 = (np.arange(tiles_y) * ty)[:, None, None]
+    mixed, waves = 0, 0
+    for d, arrs in ((0, (xp, xn)), (1, (ep, en))):
+        if d == 0:
+            i, j = X0 + t % (tx + 1), Y0 + t // (tx + 1)
+            valid = (t < (tx + 1) * ty) & (i <= ni) & (j < nj)
+        else:
+            i, j = X0 + t % tx, Y0 + t // tx
+            valid = (t < tx * (ty + 1)) & (i < ni) & (j <= nj)
+        ic, jc = np.minimum(i, ni if d == 0 else ni - 1), np.minimum(j, nj - 1 if d == 0 else nj)
+        for a in arrs:
+            v = a[jc, ic] if d == 0 else a[ic, jc]
+            ch = (v <= delta) & valid
+            cw = (v > delta) & valid
+            shp = ch.shape[:2] + (nt // 64, 64)
+            anych, anycw = ch.reshape(shp).any(-1), cw.reshape(shp).any(-1)
+            mixed += int(np.count_nonzero(anych & anycw))
+            waves += int(np.count_nonzero(anych | anycw))
+    return nchar / npairs, mixed / waves
+
+
+def adaweno_leg(delta: float = 0.5, reps: int = 2):
+    """AdaWENO (105) on developed flows: Rayleigh-Taylor 1024x4096 run to
+    t = 1.0 (the mixing layer) and double Mach 1920x480 run to t = 0.1.  At
+    each state: the fraction of (face, sign) pairs on the characteristic
+    branch (theta <= delta, dense_get_theta), the fraction of wavefronts that
+    mix both branches, us per stage launch at delta = 0 (all component-wise),
+    the run's delta and 1 (all characteristic) from the same state, and the
+    divergence cost: the measured time over the time a perfectly sorted mix
+    of the two branches would take, T0 + f (T1 - T0)."""
+    D = importlib.import_module("2dense_amd")
+    cases = [("rt", "Rayleigh-Taylor 1024x4096 at t = 1.0", D.make_cfg(1024, 4096, benchmark=3, splitting=105,
+                                                                       recon=255, cfl=0.5, delta=delta),
+              rt_initial_state(1024, 4096), 1.0),
+             ("dmr", "double Mach 1920x480 at t = 0.1", D.make_cfg(NI, NJ, benchmark=5, splitting=105, recon=255,
+                                                                  cfl=CFL, delta=delta),
+              dmr_initial_state(NI, NJ), 0.1)]
+    out = []
+    for key, name, cfg, ic, t_dev in cases:
+        with D.Solver(cfg) as s:
+            s.set_state(ic)
+            n_dev = s.run(10 ** 7, total_time=t_dev)
+            th = s.theta()
+            frac, mixed = _branch_stats(th, delta, cfg.ni, cfg.nj)
+            state, t0, st0 = s.interior(), s.current_time, s.current_step
+            us = {}
+            for d in (0.0, delta, 1.0):
+                s.update_params(delta=d)
+                best = None
+                for _ in range(reps + 1):
+                    s.set_state(state)
+                    s.current_time, s.current_step = t0, st0
+                    s.run(16)
+                    ms, n = s.last_run_gpu_ms()
+                    best = ms / n * 1e3 if best is None else min(best, ms / n * 1e3)
+                us[d] = best
+        ideal = us[0.0] + frac * (us[1.0] - us[0.0])
+        row = dict(workload=name, steps_to_state=n_dev, char_pair_fraction=frac, mixed_wave_fraction=mixed,
+                   us_per_launch=us[delta], us_all_componentwise=us[0.0], us_all_characteristic=us[1.0],
+                   divergence_cost=us[delta] / ideal - 1.0,
+                   cell_updates_per_s=cfg.ni * cfg.nj / (3 * us[delta] * 1e-6))
+        f0, f1 = ADAWENO_FLOPS[key]
+        flops = f0 + frac * (f1 - f0)
+        row.update(flops_per_cell_update=flops,
+                   algorithmic_tflops=flops / 3.0 * cfg.ni * cfg.nj / (us[delta] * 1e-6) / 1e12)
+        out.append(row)
+    return out
+
+
+def _host_info() -> dict:
+    info = dict(nproc=os.cpu_count())
+    try:
+        info["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    info["model"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return info
+
+
 def _threads_available() -> int:
+    """Host threads for the CPU baseline: the cores this process may use, at
+    most the 16 host cores a one-GPU lease is allotted (the GPU box shows more
+    cores than the lease owns)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
@@ -347,7 +519,7 @@ def cpu_leg(threads: int, steps: int):
             return dict(value=NI * NJ * steps / cpu_t, unit="cell-updates/s", cores=threads, kind="reference",
                         sample=f"reference solver (flang -O2 -fopenmp, eta-sweep race fixed), DMR {NI}x{NJ} "
                                f"switch {SPLITTING}, {steps} steps, {threads} OpenMP threads, "
-                               f"timed by its own CpuTime.dat ({cpu_t:.2f} s)")
+                               f"timed by its own CpuTime.dat ({cpu_t:.2f} s)", host=_host_info())
         finally:
             shutil.rmtree(wd, ignore_errors=True)
     # fallback: the C restatement (bitwise equal to the reference)
@@ -363,17 +535,12 @@ def cpu_leg(threads: int, steps: int):
                 sample=f"C restatement, DMR {NI}x{NJ} switch {SPLITTING}, {steps} steps, {threads} threads")
 
 
-def copy_probe(nbytes: int):
-    """One plain device copy of nbytes, to calibrate the HBM counters: the
-    FETCH_SIZE/WRITE_SIZE a profiler reports for it should equal nbytes."""
-    import torch
-    n = nbytes // 8
-    src = torch.arange(n, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
-    torch.cuda.synchronize()
-    dst.copy_(src)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes": n * 8}))
+def hbm_probe(nbytes: int):
+    """The streaming read and copy kernels of dense_probe_hbm over nbytes, to
+    calibrate the HBM counters: per dispatch, read_probe_kernel fetches nbytes
+    and copy_probe_kernel fetches and writes nbytes each."""
+    D = importlib.import_module("2dense_amd")
+    print(json.dumps(D.probe_peaks(nbytes)))
 
 
 def read_traffic():
@@ -397,11 +564,13 @@ def main():
     ap.add_argument("--no-workloads", action="store_true")
     ap.add_argument("--scheme-table", action="store_true",
                     help="only time every switch pair on double Mach 1920x480 (one JSON line per pair)")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
-                    help="only run one plain device-to-device copy of BYTES (HBM counter calibration)")
+    ap.add_argument("--no-driver", action="store_true")
+    ap.add_argument("--no-adaweno", action="store_true")
+    ap.add_argument("--hbm-probe", type=int, default=0, metavar="BYTES",
+                    help="only run the streaming read/copy probe over BYTES (HBM counter calibration)")
     args = ap.parse_args()
-    if args.copy_probe:
-        copy_probe(args.copy_probe)
+    if args.hbm_probe:
+        hbm_probe(args.hbm_probe)
         return
     if args.scheme_table:
         for row in scheme_table():
@@ -420,10 +589,19 @@ def main():
     ms_launch = gpu_ms / max(launches, 1)
     flops_launch = FLOPS_PER_CELL_UPDATE * NI * NJ / 3.0
     achieved = flops_launch / (ms_launch * 1e-3)
-    roofline = dict(bound="fp64-valu", achieved=achieved / 1e12, peak=PEAK_FP64_VALU / 1e12, unit="TFLOP/s",
-                    frac=achieved / PEAK_FP64_VALU, traffic=read_traffic(),
+    peaks = probe_leg()
+    # the roofline's peak: the lower of MI355X's published FP64 rate and the
+    # FMA-loop rate measured on this device
+    peak = min(PEAK_FP64_VALU, peaks["fp64_tflops"] * 1e12)
+    hbm_peak = min(PEAK_HBM, peaks["hbm_read_gbs"] * 1e9)
+    hbm_alg = BYTES_PER_CELL_UPDATE / 3.0 * NI * NJ / (ms_launch * 1e-3)
+    roofline = dict(bound="fp64-valu", achieved=achieved / 1e12, peak=peak / 1e12, unit="TFLOP/s",
+                    frac=achieved / peak, traffic=read_traffic(),
+                    peak_published=PEAK_FP64_VALU / 1e12, peak_measured=peaks["fp64_tflops"],
                     ms_per_launch=ms_launch, flops_per_launch=flops_launch,
-                    hbm_algorithmic_gbs=BYTES_PER_CELL_UPDATE / 3.0 * NI * NJ / (ms_launch * 1e-3) / 1e9)
+                    hbm_algorithmic_gbs=hbm_alg / 1e9, hbm_peak_gbs=hbm_peak / 1e9, hbm_frac=hbm_alg / hbm_peak,
+                    hbm_measured=dict(read_gbs=peaks["hbm_read_gbs"], copy_gbs=peaks["hbm_copy_gbs"],
+                                      bytes=peaks["hbm_bytes"]))
     batched = None
     if not args.no_batched:
         batched = batched_leg()
@@ -433,6 +611,12 @@ def main():
     workloads = None
     if not args.no_workloads:
         workloads = workloads_leg()
+    adaweno = None
+    if not args.no_adaweno:
+        adaweno = adaweno_leg()
+    driver = None
+    if not args.no_driver:
+        driver = driver_leg()
     cpu = None
     if not args.no_cpu:
         try:
@@ -460,6 +644,8 @@ def main():
         "batched": batched,
         "adjoint": adjoint,
         "other_workloads": workloads,
+        "adaweno_developed": adaweno,
+        "driver": driver,
     }
     if cpu and cpu.get("value"):
         out["speedup_vs_cpu"] = value / cpu["value"]

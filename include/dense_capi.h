@@ -21,6 +21,8 @@
 #ifndef DENSE_CAPI_H
 #define DENSE_CAPI_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -196,6 +198,15 @@ int dense_autotune(dense_handle h, int trial_steps, int* tile, double* best_ms);
  * theta_xip(ni+1,nj), theta_xin(ni+1,nj), theta_etap(nj+1,ni),
  * theta_etan(nj+1,ni), Fortran order, one after the other. */
 int dense_get_theta(dense_handle h, double* theta);
+
+/* Roofline calibration of the current device (no counterpart in the reference):
+ * dense_probe_fp64 times a loop of independent FP64 FMAs over every CU and
+ * returns the achieved TFLOP/s (the vector FP64 peak the stage kernels are
+ * bound by); dense_probe_hbm times a streaming read and a streaming copy of
+ * `bytes` bytes (>= 1 MiB; use sizes well beyond the 256 MB last-level cache)
+ * and returns GB/s (the copy counts its reads and writes).  Best of 5 runs. */
+int dense_probe_fp64(double* tflops);
+int dense_probe_hbm(size_t bytes, double* read_gbs, double* copy_gbs);
 
 /* Wait for queued device work of this handle. */
 int dense_sync(dense_handle h);

@@ -8,7 +8,13 @@ counted; abs/neg/max/compare count zero.  The count includes the reference's
 own boundary overhead (fluxes on the ghost columns, ni+1 faces per row), so
 it is quoted at the workload's grid size.
 
-    python oracle/count_ops.py 5 1920 480 104 255
+    python oracle/count_ops.py 5 1920 480 104 255 [delta]
+
+For AdaWENO (105) the count depends on how many (face, sign) pairs take the
+characteristic branch (theta <= delta): delta = 0 sends every pair down the
+component-wise branch and delta = 1 every pair down the characteristic one
+(theta <= 1 always), and the reference's cost is linear in the fraction of
+characteristic pairs between those two counts.
 """
 import json
 import sys
@@ -16,9 +22,9 @@ import sys
 import pyoracle as po
 
 
-def count(benchmark, ni, nj, splitting, recon, steps=1, warm=0):
+def count(benchmark, ni, nj, splitting, recon, steps=1, warm=0, delta=0.5):
     p = po.problem_params(benchmark, ni, nj, splitting=splitting, recon=recon,
-                          cfl=0.6 if benchmark == 5 else 0.5, ops=True)
+                          cfl=0.6 if benchmark == 5 else 0.5, delta=delta, ops=True)
     o = po.Oracle(p, ops=True)
     o.init_state(benchmark)
     o.step(warm)
@@ -31,5 +37,6 @@ def count(benchmark, ni, nj, splitting, recon, steps=1, warm=0):
 
 if __name__ == "__main__":
     bm, ni, nj, sp, rc = (int(v) for v in sys.argv[1:6])
-    print(json.dumps({"benchmark": bm, "ni": ni, "nj": nj, "splitting": sp, "recon": rc,
-                      "flops_per_cell_update": count(bm, ni, nj, sp, rc)}))
+    delta = float(sys.argv[6]) if len(sys.argv) > 6 else 0.5
+    print(json.dumps({"benchmark": bm, "ni": ni, "nj": nj, "splitting": sp, "recon": rc, "delta": delta,
+                      "flops_per_cell_update": count(bm, ni, nj, sp, rc, delta=delta)}))

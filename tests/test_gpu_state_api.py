@@ -182,3 +182,16 @@ def test_theta_is_one_on_a_uniform_state_and_rejected_elsewhere():
             s.step(1)
             with pytest.raises(D.DenseError):
                 s.theta()
+
+
+def test_roofline_probes_report_plausible_rates():
+    """dense_probe_fp64 / dense_probe_hbm (the measured denominators of the
+    bench's roofline): MI355X publishes 78.6 TFLOP/s FP64 and 8 TB/s, and a
+    sustained FMA loop or stream runs below the published figure but within a
+    factor of two of it."""
+    D = dense()
+    p = D.probe_peaks(1 << 30)
+    print(p, flush=True)
+    assert 39.3 <= p["fp64_tflops"] <= 78.6 * 1.05
+    assert 4000 <= p["hbm_read_gbs"] <= 8000 * 1.05
+    assert 2000 <= p["hbm_copy_gbs"] <= 8000 * 1.05
