@@ -329,11 +329,12 @@ def probe_leg():
     return D.probe_peaks(2 << 30)
 
 
-def driver_leg(steps: int = 160):
+def driver_leg(total_time: float = 0.2):
     """The Fortran drop-in driver (2dense_amd/bin/dense_gpu, the reference's
-    input.in and loop, general_main.f90:23-38) on the headline workload, timed
-    by its own CpuTime.dat as the reference times its loop.  It replays the
-    steps between saves from captured graphs (dense_run_saving)."""
+    input.in and loop, general_main.f90:23-38) on the headline workload as
+    configured, to t = 0.2, timed by its own CpuTime.dat as the reference
+    times its loop (graph captures at the first steps included).  It replays
+    the steps between saves from captured graphs (dense_run_saving)."""
     sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
     drv = os.path.join(ROOT, "2dense_amd", "bin", "dense_gpu")
     if not os.path.exists(drv):
@@ -342,21 +343,22 @@ def driver_leg(steps: int = 160):
     try:
         with open(os.path.join(wd, "input.in"), "w") as f:
             f.write(_input_in(benchmark=5, ni=NI, nj=NJ, splitting=SPLITTING, recon=RECON, cfl=CFL,
-                              total_step=steps, save_step=10 ** 8, threads=1))
+                              total_step=10 ** 8, save_step=10 ** 8, threads=1, total_time=total_time))
         env = dict(os.environ, DENSE_AUTOTUNE="0")
         out = subprocess.run([drv], cwd=wd, env=env, capture_output=True, text=True, timeout=600)
         if out.returncode != 0:
             return dict(error=out.stdout[-500:] + out.stderr[-500:])
         with open(os.path.join(wd, "CpuTime.dat")) as f:
-            line = f.readline()
-        t = float(line.split(":")[1].split()[0].replace("D", "E"))
-        return dict(workload=f"double Mach {NI}x{NJ}, switch {SPLITTING}, {steps} steps through the Fortran driver",
-                    steps=steps, loop_s=t, cell_updates_per_s=NI * NJ * steps / t)
+            fields = f.readline().split(":")[1].split()
+        t, steps = float(fields[0].replace("D", "E")), int(fields[1])
+        return dict(workload=f"double Mach {NI}x{NJ}, switch {SPLITTING}, CFL {CFL}, to t = {total_time} through the "
+                             "Fortran driver", steps=steps, loop_s=t, cell_updates_per_s=NI * NJ * steps / t)
     finally:
         shutil.rmtree(wd, ignore_errors=True)
 
 
-def _input_in(*, benchmark, ni, nj, splitting, recon, cfl, total_step, save_step, threads, gamma=1.4):
+def _input_in(*, benchmark, ni, nj, splitting, recon, cfl, total_step, save_step, threads, gamma=1.4,
+              total_time=1.0e30):
     """The reference's six NAMELIST groups (flow_mod_general.f90:313-332) with
     output off: no dumps, no Tecplot saves before the end of the run."""
     return f"""&OMP_PARS
@@ -370,7 +372,7 @@ def _input_in(*, benchmark, ni, nj, splitting, recon, cfl, total_step, save_step
  IF_CURVE_GRID=F, IF_READ_FROM_FILE=F, SINGLE_NI={ni}, SINGLE_NJ={nj},
  /
 &TIME_STEP_PARS
- STAGES=3, TOTAL_TIME=1.0e30, DT_SAVE=1.0e30, DT=0.01, CFL={cfl!r},
+ STAGES=3, TOTAL_TIME={total_time!r}, DT_SAVE=1.0e30, DT=0.01, CFL={cfl!r},
  TOTAL_STEP={total_step}, SAVE_STEP={save_step},
  IF_CONST_DT=F, IF_SAVE_BINARY=F, IF_SAVE_TEC=F,
  /
