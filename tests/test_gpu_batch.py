@@ -167,3 +167,33 @@ def test_device_initial_states_reject_bad_descriptors():
         b.init_ic(good)
         b.run(2)
         assert np.all(np.isfinite(b.interior(0)))
+
+
+@pytest.mark.parametrize("splitting,recon", [(101, 251), (101, 255), (104, 255)])
+def test_batch_with_large_instances_on_512_thread_tiles(splitting, recon):
+    """An ensemble whose largest instances (512^2 and a 512x256 shock/vortex)
+    hold far more tiles than the persistent grid has workgroups, on the 32x15
+    schedule with 512 threads: each workgroup pulls many tiles across
+    instance boundaries, and (for upwind5) writes its face fluxes over the
+    cell records.  Every instance equals its single-block run bit for bit."""
+    D = dense()
+    B = _batch_mod()
+    big_v = B.vortex_state(512, 3.5, 2.0, 7.0)
+    big_sv = B.shock_vortex_state(512, 256, 1.3, 0.25, 0.4)
+    cfgs, states, _ = _ensemble(n=4, seed=9, splitting=splitting, recon=recon)
+    cfgs = list(cfgs) + [
+        D.make_cfg(512, 512, benchmark=0, splitting=splitting, recon=recon, cfl=0.5, gamma=1.4, dx=big_v[1],
+                   dy=big_v[1], bc=[D.BC_PERIODIC] * 4),
+        D.make_cfg(512, 256, benchmark=0, splitting=splitting, recon=recon, cfl=0.5, gamma=1.4, dx=big_sv[1],
+                   dy=big_sv[2], bc=[D.BC_EXTRAP, D.BC_EXTRAP, D.BC_REFLECT, D.BC_REFLECT])]
+    states = list(states) + [big_v[0], big_sv[0]]
+    steps = 5
+    print(f"batch {splitting}/{recon} with 512^2 and 512x256 instances, tile 3", flush=True)
+    with B.Batch(cfgs) as b:
+        b.set_schedule(3, 1, 100)  # 32x15 with 512 threads, largest instance first
+        for i, c in enumerate(states):
+            b.set_state(i, c)
+        b.run(steps)
+        for i in range(len(cfgs)):
+            ref, _ = _alone(cfgs[i], states[i], steps)
+            assert np.array_equal(b.interior(i), ref), f"instance {i} ({cfgs[i].ni}x{cfgs[i].nj})"
