@@ -51,6 +51,22 @@ __device__ __forceinline__ double fdiv(double a, double b)
     return fma(fma(-b, q0, a), y, q0);
 }
 
+// sqrt(x) and 1/sqrt(x) of a positive normal x from v_rsq_f64, refined by the
+// coupled Newton iteration for (sqrt(x), 1/(2 sqrt(x))) and a last residual
+// step on sqrt(x): within an ulp, without the generic square root's scaling
+// and special-case code (x here is a density or a squared sound speed).
+__device__ __forceinline__ void sqrt_rsq(double x, double& sq, double& rs)
+{
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r, h = 0.5 * r;
+    const double e = fma(-g, h, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    const double d = fma(-g, g, x);
+    sq = fma(d, h, g);
+    rs = 2.0 * h;
+}
+
 // ---------------------------------------------------------------- stencils
 
 // smoothness indicators, weno_mod.f90:152-159 / :175-182:
@@ -79,7 +95,8 @@ __device__ __forceinline__ void candidates(const double* lf, double& q0, double&
 }
 
 // WENO-Z unnormalised weights a_k = c_k (1 + (tau5/(eps + beta_k))^2),
-// tau5 = |beta_2 - beta_0| (weno_mod.f90:184-188).
+// tau5 = |beta_2 - beta_0| (weno_mod.f90:184-188), for the indicator and
+// theta of AdaWENO (which need the sum of these weights itself).
 __device__ __forceinline__ void weno5z_alpha(const double* lf, double& aa0, double& aa1, double& aa2)
 {
     double is0, is1, is2;
@@ -110,24 +127,36 @@ __device__ __forceinline__ void weno5z_omega(const double* lf, double& w0, doubl
     }
 }
 
-// weno5z_stencil, weno_mod.f90:50-65: sum w_k q_k = (sum a_k q_k) / sum a_k,
-// one reciprocal per stencil
+// weno5z_stencil, weno_mod.f90:50-65: sum w_k q_k with w_k = a_k / sum a_k.
+// The weights are scale-free, so every a_k is multiplied by (d0 d1 d2)^2,
+// d_k = eps + beta_k:  a_k (d0 d1 d2)^2 = c_k (d_k^2 + tau5^2) e_k^2 with
+// e_k = d0 d1 d2 / d_k the product of the other two -- no quotient left but
+// the one normalisation.  d_k >= eps = 1e-6 and beta_k is a squared flux
+// difference, so the products stay far inside the FP64 range.
 __device__ __forceinline__ double weno5z(const double* lf)
 {
-    double q0, q1, q2, aa0, aa1, aa2;
+    double q0, q1, q2, is0, is1, is2;
     candidates(lf, q0, q1, q2);
-    weno5z_alpha(lf, aa0, aa1, aa2);
+    smooth_ind(lf, is0, is1, is2);
+    const double tau5 = fabs(is2 - is0), t2 = tau5 * tau5;
+    const double d0 = WENO_SS + is0, d1 = WENO_SS + is1, d2 = WENO_SS + is2;
+    const double e0 = d1 * d2, e1 = d0 * d2, e2 = d0 * d1;
+    const double aa0 = C30 * (d0 * d0 + t2) * (e0 * e0);
+    const double aa1 = C31 * (d1 * d1 + t2) * (e1 * e1);
+    const double aa2 = C32 * (d2 * d2 + t2) * (e2 * e2);
     return (aa0 * q0 + aa1 * q1 + aa2 * q2) * rcp(aa0 + aa1 + aa2);
 }
 
-// weno5js_stencil, weno_mod.f90:98-165: a_k = c_k/(eps+beta_k)^2
+// weno5js_stencil, weno_mod.f90:98-165: a_k = c_k/(eps+beta_k)^2, scaled by
+// (d0 d1 d2)^2 as above: c_k e_k^2, one reciprocal per stencil.
 __device__ __forceinline__ double weno5js(const double* lf)
 {
     double q0, q1, q2, is0, is1, is2;
     candidates(lf, q0, q1, q2);
     smooth_ind(lf, is0, is1, is2);
-    const double r0 = rcp(WENO_SS + is0), r1 = rcp(WENO_SS + is1), r2 = rcp(WENO_SS + is2);
-    const double aa0 = C30 * (r0 * r0), aa1 = C31 * (r1 * r1), aa2 = C32 * (r2 * r2);
+    const double d0 = WENO_SS + is0, d1 = WENO_SS + is1, d2 = WENO_SS + is2;
+    const double e0 = d1 * d2, e1 = d0 * d2, e2 = d0 * d1;
+    const double aa0 = C30 * (e0 * e0), aa1 = C31 * (e1 * e1), aa2 = C32 * (e2 * e2);
     return (aa0 * q0 + aa1 * q1 + aa2 * q2) * rcp(aa0 + aa1 + aa2);
 }
 

@@ -94,8 +94,9 @@ __device__ __forceinline__ void face_cw_sw(const double* __restrict__ s, int bas
 //   AdaWENO indicator rho+F_n, theta scheme_weno5th_SC.f90:1-64, flux_splitting_SC.f90:58-59
 //   shared-weight branch (theta>delta) scheme_weno5th_SC.f90:110-124
 // The eigenvectors are written in the frame (normal, tangential); the Roe
-// average and the eigenvector entries share the reciprocals of rho_L, rho_R,
-// sqrt(rho_L)+sqrt(rho_R) and the Roe sound speed.
+// average and the eigenvector entries share the reciprocals of rho_L, rho_R
+// and 1 + sqrt(rho_L/rho_R), and the Roe sound speed and its reciprocal come
+// from one reciprocal square root.
 
 // Roe state and left eigenvectors of one face.  Chars 1, 2 and 4 have entries
 // for (rho, n, t, E) in Lr/Ln/Lt/Le[0..2]; char 3 is (-u_t, [0|1], [1|0], 0).
@@ -112,19 +113,23 @@ __device__ __forceinline__ void roe_face(const double* __restrict__ s, int base,
     const double rhor = s[oR], ur = s[REC_U * NC + oR], vr = s[REC_V * NC + oR], pr = s[REC_P * NC + oR];
     // enthalpy H = p/(gamma-1)/rho + (u^2+v^2)/2 + p/rho = gamma/(gamma-1) p/rho + (u^2+v^2)/2
     const double gm1 = gamma - 1.0, gg1 = gamma * rcp(gm1);
-    const double hl = gg1 * (pl * rcp(rhol)) + 0.5 * (ul * ul + vl * vl);
-    const double hr = gg1 * (pr * rcp(rhor)) + 0.5 * (ur * ur + vr * vr);
-    const double sr = sqrt(rhor), sl = sqrt(rhol);
-    const double isum = rcp(sr + sl);
-    const double u0 = (sr * ur + sl * ul) * isum;
-    const double v0 = (sr * vr + sl * vl) * isum;
-    R.h0 = (sr * hr + sl * hl) * isum;
-    R.a0 = sqrt(gm1 * (R.h0 - 0.5 * (u0 * u0 + v0 * v0)));
+    const double irl = rcp(rhol), irr = rcp(rhor);
+    const double hl = gg1 * (pl * irl) + 0.5 * (ul * ul + vl * vl);
+    const double hr = gg1 * (pr * irr) + 0.5 * (ur * ur + vr * vr);
+    // Roe weights sqrt(rho_R)/(sqrt(rho_R)+sqrt(rho_L)) = 1/(1+t) and t/(1+t),
+    // t = sqrt(rho_L/rho_R): one square root instead of two
+    double t, rt;
+    sqrt_rsq(rhol * irr, t, rt);
+    const double wr = rcp(1.0 + t), wl = t * wr;
+    const double u0 = wr * ur + wl * ul;
+    const double v0 = wr * vr + wl * vl;
+    R.h0 = wr * hr + wl * hl;
+    double ia;  // a0 and 1/a0 from one reciprocal square root
+    sqrt_rsq(gm1 * (R.h0 - 0.5 * (u0 * u0 + v0 * v0)), R.a0, ia);
     R.un0 = DIR == 0 ? u0 : v0;
     R.ut0 = DIR == 0 ? v0 : u0;
     R.uu0 = 0.5 * (u0 * u0 + v0 * v0);
     // 1/hh0 = (gamma-1)/a0^2, with hh0 = a0^2/(gamma-1) (scheme_weno5th_SC.f90:277-328)
-    const double ia = rcp(R.a0);
     const double ihh = gm1 * (ia * ia);
     const double A = 0.5 * R.uu0 * ihh, B = 0.5 * R.un0 * ia, Cc = 0.5 * R.un0 * ihh, D = 0.5 * ia;
     const double E = 0.5 * R.ut0 * ihh, F = 0.5 * ihh;
