@@ -224,14 +224,21 @@ __device__ __forceinline__ void cw_shared(const double F[5][4], double w0, doubl
 }
 
 // AdaWENO's indicator of one sign: WENO-Z weights of rho + F_n on the sign's
-// stencil, and the branch: characteristic unless theta > delta (strict).
+// stencil, and the branch: characteristic unless theta > delta (strict), with
+// theta = 1/(1 + (S - 1)^2), S the sum of the unnormalised WENO-Z weights
+// (weno_mod.f90:94, scheme_weno5th_SC.f90:24-28, :110).  For 0 < delta < 1,
+// theta > delta is (S - 1)^2 < (1 - delta)/delta; with the weights scaled by
+// P^2 = (d0 d1 d2)^2 as in weno5z (S' = S P^2) that is
+// (S' - P^2)^2 < (1 - delta)/delta P^4: no quotient, and one reciprocal for
+// the shared weights w_k = a'_k / S'.  theta <= 1 always, so delta >= 1 takes
+// the characteristic branch everywhere and delta <= 0 the shared-weight one.
 template <int DIR, int NC, int ST>
 __device__ __forceinline__ bool adaweno_sign(const double* __restrict__ s, int base, bool plus, double alpha,
                                             double delta, double& w0, double& w1, double& w2)
 {
     constexpr int NRM = DIR == 0 ? 1 : 2;
     constexpr int VN = DIR == 0 ? REC_U : REC_V;
-    double lfi[5], th;
+    double lfi[5];
     const double sa = plus ? alpha : -alpha;
 #pragma unroll
     for (int i = 0; i < 5; i++) {
@@ -239,8 +246,31 @@ __device__ __forceinline__ bool adaweno_sign(const double* __restrict__ s, int b
         const double un = s[VN * NC + o];
         lfi[i] = s[o] + 0.5 * ((un + sa) * s[NRM * NC + o] + s[REC_P * NC + o]);
     }
-    weno5z_omega(lfi, w0, w1, w2, &th);
-    return !(th > delta);
+    double is0, is1, is2;
+    smooth_ind(lfi, is0, is1, is2);
+    const double tau5 = fabs(is2 - is0), t2 = tau5 * tau5;
+    const double d0 = WENO_SS + is0, d1 = WENO_SS + is1, d2 = WENO_SS + is2;
+    const double e0 = d1 * d2, e1 = d0 * d2, e2 = d0 * d1;
+    const double aa0 = C30 * (d0 * d0 + t2) * (e0 * e0);
+    const double aa1 = C31 * (d1 * d1 + t2) * (e1 * e1);
+    const double aa2 = C32 * (d2 * d2 + t2) * (e2 * e2);
+    const double sw = aa0 + aa1 + aa2;
+    const double p = e0 * d0, p2 = p * p;
+    bool shared;
+    if (delta >= 1.0) shared = false;
+    else if (delta <= 0.0) shared = true;
+    else {
+        const double k = (1.0 - delta) * rcp(delta);
+        const double m = sw - p2;
+        shared = m * m < k * (p2 * p2);
+    }
+    if (shared) {
+        const double ys = rcp(sw);
+        w0 = aa0 * ys;
+        w1 = aa1 * ys;
+        w2 = aa2 * ys;
+    }
+    return !shared;
 }
 
 // Characteristic WENO-Z (switch 104) and AdaWENO (switch 105) at one face:
