@@ -7,17 +7,13 @@
 
 namespace dense {
 
-// Characteristic WENO-Z (switch 104) reconstructs its two flux signs one after
-// the other, at 3 waves per SIMD (167 VGPRs, 76 B/lane of spill): 82 us per
-// stage launch on double Mach 1920x480 against 90 us with the signs
-// interleaved at 2 waves (243 VGPRs).  AdaWENO (105), which holds its
-// indicator weights across both signs, stays interleaved at 2 waves (serial at
-// 3 waves measured 7 % slower on double Mach, 3 % on Rayleigh-Taylor).
-template <bool ADAPT>
-constexpr bool sign_serial()
-{
-    return !ADAPT;
-}
+// The characteristic faces (switch 104 and 105) reconstruct their two flux
+// signs one after the other at 3 waves per SIMD: 104 at 168 VGPRs with
+// 52 B/lane of spill, 73.6 us per stage launch on double Mach 1920x480,
+// against 90 us with the signs interleaved at 2 waves (243 VGPRs); AdaWENO
+// 200 vs 215 us on Rayleigh-Taylor 1024x4096 and 63 vs 69 us on double Mach
+// at t = 0.1, where serial signs also let each sign's indicator alone decide
+// whether the face's Roe state is built.
 
 // LDS record slots of one cell: 0..3 conserved (rho, rho u, rho v, E), then u, v, p, c.
 enum { REC_U = 4, REC_V = 5, REC_P = 6, REC_C = 7, REC_N = 8 };
@@ -273,35 +269,58 @@ __device__ __forceinline__ bool adaweno_sign(const double* __restrict__ s, int b
     return !shared;
 }
 
-// Characteristic WENO-Z (switch 104) and AdaWENO (switch 105) at one face:
-// h(face) = h+ + h- into h[0..3] (h[4..7] scratch).
-template <int DIR, bool ADAPT, int NC, int ST>
+// Characteristic WENO-Z (switch 104) at one face: h(face) = h+ + h- into
+// h[0..3] (h[4..7] scratch), the signs one after the other.
+template <int DIR, int NC, int ST>
 __device__ __forceinline__ void face_char(const double* __restrict__ s, int base, double alpha, double gamma,
-                                          double delta, double* h)
+                                          double* h)
 {
-    // --- AdaWENO indicator weights per sign
-    bool chp = true, chm = true;
-    double wp0 = 0, wp1 = 0, wp2 = 0, wm0 = 0, wm1 = 0, wm2 = 0;
-    if constexpr (ADAPT) {
-        chp = adaweno_sign<DIR, NC, ST>(s, base, true, alpha, delta, wp0, wp1, wp2);
-        chm = adaweno_sign<DIR, NC, ST>(s, base, false, alpha, delta, wm0, wm1, wm2);
-    }
-    // --- Roe state and eigenvectors (only when a sign needs them)
     RoeFace R;
-    if (chp || chm) roe_face<DIR, NC, ST>(s, base, gamma, R);
-
-    // One sign per iteration: h[0..3] for F+, h[4..7] for F-.  Serial signs
-    // (fewer live registers) or both unrolled and interleaved by the scheduler
-    // (more ILP per wave), see sign_serial.
-    constexpr int SIGN_UNROLL = sign_serial<ADAPT>() ? 1 : 2;
-#pragma unroll SIGN_UNROLL
+    roe_face<DIR, NC, ST>(s, base, gamma, R);
+#pragma unroll 1
     for (int sg = 0; sg < 2; sg++) {
         const bool plus = sg == 0;
-        double F[5][4];
+        double F[5][4], r4[4];
         flux_stencil<DIR, NC, ST>(s, base, plus, alpha, F);
-        double r4[4];
-        if (plus ? chp : chm) char_recon<DIR>(F, R, r4);
-        else cw_shared(F, plus ? wp0 : wm0, plus ? wp1 : wm1, plus ? wp2 : wm2, r4);
+        char_recon<DIR>(F, R, r4);
+        if (plus) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) h[k] = r4[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++) h[4 + k] = r4[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) h[k] = h[k] + h[4 + k];
+}
+
+// AdaWENO (switch 105) at one face, the signs one after the other: each
+// sign's indicator decides its branch (scheme_weno5th_SC.f90:110), and the
+// Roe state is built at the first sign that takes the characteristic branch
+// (once per face at most; most faces of developed flows never need it).
+template <int DIR, int NC, int ST>
+__device__ __forceinline__ void face_ada(const double* __restrict__ s, int base, double alpha, double gamma,
+                                         double delta, double* h)
+{
+    RoeFace R;
+    bool have_roe = false;
+#pragma unroll 1
+    for (int sg = 0; sg < 2; sg++) {
+        const bool plus = sg == 0;
+        double w0 = 0, w1 = 0, w2 = 0;
+        const bool ch = adaweno_sign<DIR, NC, ST>(s, base, plus, alpha, delta, w0, w1, w2);
+        double F[5][4], r4[4];
+        flux_stencil<DIR, NC, ST>(s, base, plus, alpha, F);
+        if (ch) {
+            if (!have_roe) {
+                roe_face<DIR, NC, ST>(s, base, gamma, R);
+                have_roe = true;
+            }
+            char_recon<DIR>(F, R, r4);
+        } else {
+            cw_shared(F, w0, w1, w2, r4);
+        }
         if (plus) {
 #pragma unroll
             for (int k = 0; k < 4; k++) h[k] = r4[k];

@@ -173,8 +173,8 @@ template <int SCHEME, int DIR, int NC, int ST>
 __device__ __forceinline__ void eval_face(const double* __restrict__ s, int base, const StageArgs& a, double alpha,
                                           double* h)
 {
-    if constexpr (SCHEME == S_CH) face_char<DIR, false, NC, ST>(s, base, alpha, a.gamma, a.delta, h);
-    else if constexpr (SCHEME == S_SC) face_char<DIR, true, NC, ST>(s, base, alpha, a.gamma, a.delta, h);
+    if constexpr (SCHEME == S_CH) face_char<DIR, NC, ST>(s, base, alpha, a.gamma, h);
+    else if constexpr (SCHEME == S_SC) face_ada<DIR, NC, ST>(s, base, alpha, a.gamma, a.delta, h);
     else if constexpr (SCHEME >= S_SW_UP5) face_cw_sw<DIR, SCHEME - S_SW_UP5, NC, ST>(s, base, a.gamma, h);
     else if constexpr (SCHEME >= S_GLF_UP5) face_cw_lf<DIR, false, SCHEME - S_GLF_UP5, NC, ST>(s, base, alpha, h);
     else face_cw_lf<DIR, true, SCHEME - S_LLF_UP5, NC, ST>(s, base, alpha, h);
@@ -188,18 +188,11 @@ constexpr bool faces_read_sound_speed()
     return SCHEME <= S_LLF_Z || (SCHEME >= S_SW_UP5 && SCHEME <= S_SW_Z);
 }
 
-// Minimum waves per SIMD requested from the register allocator, per scheme:
-// 3 for characteristic WENO-Z (its signs run one after the other, see
-// sign_serial), 2 for AdaWENO, 3 for the component-wise schemes (4 for the
-// aliased upwind5 tiles below).
-#ifndef DENSE_WAVES_CH
-#define DENSE_WAVES_CH 3
-#endif
-#ifndef DENSE_WAVES_SC
-#define DENSE_WAVES_SC 2
-#endif
-#ifndef DENSE_WAVES_CW
-#define DENSE_WAVES_CW 3
+// Minimum waves per SIMD requested from the register allocator: 3 for every
+// scheme (the characteristic ones run their flux signs one after the other to
+// fit), 4 for the aliased upwind5 tiles below.
+#ifndef DENSE_WAVES
+#define DENSE_WAVES 3
 #endif
 // The Lax-Friedrichs upwind5 schemes on 512-thread tiles keep a face's h in
 // registers until every face of the tile has read the cell records, then write
@@ -221,10 +214,7 @@ constexpr bool alias_h()
 template <int SCHEME, class T>
 constexpr int min_waves()
 {
-    return SCHEME == S_CH              ? DENSE_WAVES_CH
-           : SCHEME == S_SC            ? DENSE_WAVES_SC
-           : alias_h<SCHEME, T>()      ? 4
-                                       : DENSE_WAVES_CW;
+    return alias_h<SCHEME, T>() ? 4 : DENSE_WAVES;
 }
 
 // Per-XCD tile queue of a persistent launch: queue x owns the contiguous
