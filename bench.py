@@ -554,6 +554,14 @@ def read_traffic():
     return None
 
 
+def _leg(name, fn, *a):
+    """Run one leg and report it on stderr (stdout carries only the JSON line)."""
+    t0 = time.time()
+    r = fn(*a)
+    print(f"bench.py: {name} done in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -583,7 +591,7 @@ def main():
               "(multi-GPU sharding is design-only)", file=sys.stderr)
         sys.exit(2)
 
-    taken, wall, gpu_ms, launches = gpu_leg(args.steps, args.warmup)
+    taken, wall, gpu_ms, launches = _leg("double Mach 1920x480", gpu_leg, args.steps, args.warmup)
     if taken != args.steps:
         raise RuntimeError(f"ran {taken} of {args.steps} steps")
     cu = NI * NJ * taken
@@ -591,7 +599,7 @@ def main():
     ms_launch = gpu_ms / max(launches, 1)
     flops_launch = FLOPS_PER_CELL_UPDATE * NI * NJ / 3.0
     achieved = flops_launch / (ms_launch * 1e-3)
-    peaks = probe_leg()
+    peaks = _leg("roofline probes", probe_leg)
     # the roofline's peak: the lower of MI355X's published FP64 rate and the
     # FMA-loop rate measured on this device
     peak = min(PEAK_FP64_VALU, peaks["fp64_tflops"] * 1e12)
@@ -606,23 +614,23 @@ def main():
                                       bytes=peaks["hbm_bytes"]))
     batched = None
     if not args.no_batched:
-        batched = batched_leg()
+        batched = _leg("batched", batched_leg)
     adjoint = None
     if not args.no_adjoint:
-        adjoint = adjoint_leg()
+        adjoint = _leg("adjoint", adjoint_leg)
     workloads = None
     if not args.no_workloads:
-        workloads = workloads_leg()
+        workloads = _leg("workloads", workloads_leg)
     adaweno = None
     if not args.no_adaweno:
-        adaweno = adaweno_leg()
+        adaweno = _leg("adaweno developed", adaweno_leg)
     driver = None
     if not args.no_driver:
-        driver = driver_leg()
+        driver = _leg("fortran driver", driver_leg)
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_leg(_threads_available(), args.cpu_steps)
+            cpu = _leg("cpu baseline", cpu_leg, _threads_available(), args.cpu_steps)
         except Exception as e:  # the GPU number stands without the baseline
             cpu = dict(value=None, error=str(e))
     out = {
