@@ -401,10 +401,7 @@ static StageArgs adj_stage_args(const dense_adjoint* a)
     s.if_const_dt = c.if_const_dt;
     s.tiles_x = a->tiles_x;
     s.ntiles = a->ntiles;
-    for (int q = 0; q < 4; q++) {
-        s.bc.kind[q] = c.bc[q];
-        for (int v = 0; v < 4; v++) s.bc.state[q][v] = c.bc_state[q][v];
-    }
+    bc_fill(s.bc, c.bc, c.bc_state, c.gamma);
     return s;
 }
 

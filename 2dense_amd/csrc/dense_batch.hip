@@ -446,10 +446,7 @@ extern "C" int dense_batch_create(const dense_cfg* cfgs, int n, dense_batch_hand
         I.dt_const = c.dt;
         I.if_const_dt = c.if_const_dt;
         I.if_bodyforce = c.if_bodyforce;
-        for (int s = 0; s < 4; s++) {
-            I.bc.kind[s] = c.bc[s];
-            for (int v = 0; v < 4; v++) I.bc.state[s][v] = c.bc_state[s][v];
-        }
+        bc_fill(I.bc, c.bc, c.bc_state, c.gamma);
         b->off[i] = total;
         total += 3 * 4 * I.plane;
     }

@@ -200,10 +200,7 @@ static StageArgs base_args(const dense_solver* h)
     a.if_bodyforce = c.if_bodyforce;
     a.tiles_x = h->tiles_x;
     a.ntiles = h->ntiles;
-    for (int s = 0; s < 4; s++) {
-        a.bc.kind[s] = c.bc[s];
-        for (int v = 0; v < 4; v++) a.bc.state[s][v] = c.bc_state[s][v];
-    }
+    bc_fill(a.bc, c.bc, c.bc_state, c.gamma);
     a.bc_rows = h->bc_rows;
     return a;
 }
@@ -874,10 +871,7 @@ extern "C" int dense_run_noalloc(const dense_cfg* cfg, double* u, int ld, int ns
     a.if_bodyforce = c.if_bodyforce;
     const int tile = resolve_tile(c.tile, c.splitting, (long long)c.ni * c.nj);
     a.ntiles = tile_count(tile, c.ni, c.nj, &a.tiles_x);
-    for (int k = 0; k < 4; k++) {
-        a.bc.kind[k] = c.bc[k];
-        for (int v = 0; v < 4; v++) a.bc.state[k][v] = c.bc_state[k][v];
-    }
+    bc_fill(a.bc, c.bc, c.bc_state, c.gamma);
     a.bc_rows = nullptr;
     const int scheme = scheme_of(c.splitting, c.recon);
 

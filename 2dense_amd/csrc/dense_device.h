@@ -195,18 +195,14 @@ __device__ __forceinline__ void prim_of(double gamma, const double* c, double* p
     pr[3] = WITH_C ? sqrt(gamma * p * ir) : 0.0;
 }
 
-// Dirichlet ghost from a primitive state, boundary_doublemach_general.f90:52-61
-__device__ __forceinline__ void from_state(double gamma, double rho, double u, double v, double p, double* c,
-                                           double* pr)
+// Dirichlet ghost from a ghost record (conserved state, then u, v, p, c; bc_fill)
+__device__ __forceinline__ void from_record(const double* r, double* c, double* pr)
 {
-    c[0] = rho;
-    c[1] = rho * u;
-    c[2] = rho * v;
-    c[3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v);
-    pr[0] = u;
-    pr[1] = v;
-    pr[2] = p;
-    pr[3] = sqrt(gamma * p / rho);
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        c[v] = r[v];
+        pr[v] = r[4 + v];
+    }
 }
 
 template <bool WITH_C = true>
@@ -239,7 +235,6 @@ __device__ __forceinline__ void ghost_cell(const StageArgs& a, const double* __r
     const bool low = (side == DENSE_SIDE_LEFT || side == DENSE_SIDE_BOTTOM);
     const int n = xside ? ni : nj;
     int kind = a.bc.kind[side];
-    const double* sv = a.bc.state[side];
 
     if (kind == DENSE_BC_DMR_WALL) {
         // x of the cell column, x(index_i-bfsize_i,1) = (i - 0.5d0)*dx
@@ -297,24 +292,30 @@ __device__ __forceinline__ void ghost_cell(const StageArgs& a, const double* __r
         // boundary_ssl_general.f90:55-85 at t^n: rho, p by the sign of y;
         // u = 2.5 + 0.5 tanh(2y) and exp(-y^2/10) tabulated per row by the host,
         // 0.05 cos(.) of the two forcing modes in st->bc_t (host libm)
-        const double y = ((double)(gy + 1) - 0.5) * a.dy - sv[0];
+        // (p/(gamma-1) and the two sound speeds from bc_fill)
+        const double y = ((double)(gy + 1) - 0.5) * a.dy - a.bc.state[side][0];
         const double e = a.bc_rows[a.nj + gy];
         const double v = a.st->bc_t[0] * e + a.st->bc_t[1] * e;
-        if (y >= 0.0) from_state(a.gamma, 1.6374, a.bc_rows[gy], v, 0.3327, c, pr);
-        else from_state(a.gamma, 0.3626, a.bc_rows[gy], v, 0.3327, c, pr);
+        const double u = a.bc_rows[gy];
+        const double rho = y >= 0.0 ? SSL_RHO_HI : SSL_RHO_LO;
+        c[0] = rho;
+        c[1] = rho * u;
+        c[2] = rho * v;
+        c[3] = a.bc.ssl_e + 0.5 * rho * (u * u + v * v);
+        pr[0] = u;
+        pr[1] = v;
+        pr[2] = SSL_P;
+        pr[3] = y >= 0.0 ? a.bc.ssl_c[0] : a.bc.ssl_c[1];
         return;
     }
     case DENSE_BC_DMR_TOP: {
         // boundary_doublemach_general.f90:67-80, shock at t^n
         const double x = ((double)(gx + 1) - 0.5) * a.dx;
-        if (x < (1.0 / 6.0 + (1.0 + 20.0 * t) / a.sqrt3))
-            from_state(a.gamma, sv[0], sv[1], sv[2], sv[3], c, pr);
-        else
-            from_state(a.gamma, 1.4, 0.0, 0.0, 1.0, c, pr);
+        from_record(x < (1.0 / 6.0 + (1.0 + 20.0 * t) / a.sqrt3) ? a.bc.rec[side] : a.bc.pre, c, pr);
         return;
     }
     default:  // DIRICHLET
-        from_state(a.gamma, sv[0], sv[1], sv[2], sv[3], c, pr);
+        from_record(a.bc.rec[side], c, pr);
         return;
     }
 }

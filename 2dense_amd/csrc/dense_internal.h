@@ -61,10 +61,50 @@ struct alignas(128) DevState {
     double bc_t[2];                  // clock terms of the shear-layer inflow at t^n (set by the host)
 };
 
+// Boundary rules of the four sides.  The ghost records (conserved state, then
+// u, v, p, c) are derived from the primitive states once per configuration by
+// bc_fill, so the stage kernels read them instead of dividing: per side the
+// record of state[side] (Dirichlet sides, the double-Mach wall's Dirichlet
+// part, the post-shock state of the moving-shock top), the pre-shock state
+// (1.4, 0, 0, 1) of the moving-shock top, and for the shear-layer inflow
+// p/(gamma-1) and the sound speeds of its two densities.
 struct BcDesc {
     int kind[4];
     double state[4][4];  // rho, u, v, p
+    double rec[4][8];
+    double pre[8];
+    double ssl_e, ssl_c[2];
 };
+
+// Shear-layer inflow (boundary_ssl_general.f90:55-85): the two densities
+// (y >= 0, y < 0) and the common pressure.
+constexpr double SSL_RHO_HI = 1.6374, SSL_RHO_LO = 0.3626, SSL_P = 0.3327;
+
+// Ghost record of a primitive state (boundary_doublemach_general.f90:52-61).
+inline void ghost_record(double gamma, double rho, double u, double v, double p, double* r)
+{
+    r[0] = rho;
+    r[1] = rho * u;
+    r[2] = rho * v;
+    r[3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v);
+    r[4] = u;
+    r[5] = v;
+    r[6] = p;
+    r[7] = __builtin_sqrt(gamma * p / rho);
+}
+
+inline void bc_fill(BcDesc& b, const int kind[4], const double state[4][4], double gamma)
+{
+    for (int s = 0; s < 4; s++) {
+        b.kind[s] = kind[s];
+        for (int v = 0; v < 4; v++) b.state[s][v] = state[s][v];
+        ghost_record(gamma, state[s][0], state[s][1], state[s][2], state[s][3], b.rec[s]);
+    }
+    ghost_record(gamma, 1.4, 0.0, 0.0, 1.0, b.pre);
+    b.ssl_e = SSL_P / (gamma - 1.0);
+    b.ssl_c[0] = __builtin_sqrt(gamma * SSL_P / SSL_RHO_HI);
+    b.ssl_c[1] = __builtin_sqrt(gamma * SSL_P / SSL_RHO_LO);
+}
 
 // Arguments of one RK-stage launch (passed by value).
 struct StageArgs {
