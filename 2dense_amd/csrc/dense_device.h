@@ -322,17 +322,20 @@ __device__ __forceinline__ void ghost_cell(const StageArgs& a, const double* __r
 
 // ---------------------------------------------------------------- fluxes
 
-// Lax-Friedrichs split of one cell along DIR, sign +1/-1:
-// F = 0.5*((u_n +/- alpha)*U + (0, p e_n, u_n p)),
-// flux_splitting_LLF.f90:39-47 (x), :64-72 (y); GLF flux_splitting_GLF.f90:31-43.
+// Lax-Friedrichs split of one cell along DIR, sign +1/-1, doubled:
+// 2F = (u_n +/- alpha)*U + (0, p e_n, u_n p), where the reference forms
+// F = 0.5*(...) (flux_splitting_LLF.f90:39-47 (x), :64-72 (y); GLF
+// flux_splitting_GLF.f90:31-43).  Multiplying by 0.5 is exact and commutes with
+// rounding, also inside an FMA, so the caller folds the 0.5 into the linear map
+// that consumes 2F and gets the reference's values bit for bit.
 template <int DIR>
-__device__ __forceinline__ void lf_flux(double upa, double un, double p, const double* c, double* f)
+__device__ __forceinline__ void lf_flux2(double upa, double un, double p, const double* c, double* f)
 {
     constexpr int NRM = DIR == 0 ? 1 : 2;
-    f[0] = 0.5 * (upa * c[0]);
-    f[NRM] = 0.5 * (upa * c[NRM] + p);
-    f[3 - NRM] = 0.5 * (upa * c[3 - NRM]);
-    f[3] = 0.5 * (upa * c[3] + un * p);
+    f[0] = upa * c[0];
+    f[NRM] = upa * c[NRM] + p;
+    f[3 - NRM] = upa * c[3 - NRM];
+    f[3] = upa * c[3] + un * p;
 }
 
 // Steger-Warming split of one cell along DIR, both signs,

@@ -94,8 +94,9 @@ __device__ __forceinline__ void face_cw_sw(const double* __restrict__ s, int bas
 // and 1 + sqrt(rho_L/rho_R), and the Roe sound speed and its reciprocal come
 // from one reciprocal square root.
 
-// Roe state and left eigenvectors of one face.  Chars 1, 2 and 4 have entries
-// for (rho, n, t, E) in Lr/Ln/Lt/Le[0..2]; char 3 is (-u_t, [0|1], [1|0], 0).
+// Roe state and left eigenvectors of one face, halved (they project doubled
+// split fluxes, lf_flux2).  Chars 1, 2 and 4 have entries for (rho, n, t, E)
+// in Lr/Ln/Lt/Le[0..2]; char 3 is (-u_t, [0|1], [1|0], 0), halved in char_recon.
 struct RoeFace {
     double Lr[3], Ln[3], Lt[3], Le[3];
     double ut0, un0, a0, h0, uu0;
@@ -127,15 +128,19 @@ __device__ __forceinline__ void roe_face(const double* __restrict__ s, int base,
     R.uu0 = 0.5 * (u0 * u0 + v0 * v0);
     // 1/hh0 = (gamma-1)/a0^2, with hh0 = a0^2/(gamma-1) (scheme_weno5th_SC.f90:277-328)
     const double ihh = gm1 * (ia * ia);
-    const double A = 0.5 * R.uu0 * ihh, B = 0.5 * R.un0 * ia, Cc = 0.5 * R.un0 * ihh, D = 0.5 * ia;
-    const double E = 0.5 * R.ut0 * ihh, F = 0.5 * ihh;
-    R.Lr[0] = A + B;             R.Ln[0] = -Cc - D;         R.Lt[0] = -E;              R.Le[0] = F;
-    R.Lr[1] = 1.0 - R.uu0 * ihh;  R.Ln[1] = R.un0 * ihh;    R.Lt[1] = R.ut0 * ihh;     R.Le[1] = -ihh;
-    R.Lr[2] = A - B;             R.Ln[2] = -Cc + D;         R.Lt[2] = -E;              R.Le[2] = F;
+    // entries of the eigenvectors times 0.5 (exact): 0.5 -> 0.25 in the rows of
+    // chars 1 and 4, (1, u_n, u_t, -1) -> (0.5, 0.5 u_n, 0.5 u_t, -0.5) in char 2
+    const double A = 0.25 * R.uu0 * ihh, B = 0.25 * R.un0 * ia, Cc = 0.25 * R.un0 * ihh, D = 0.25 * ia;
+    const double E = 0.25 * R.ut0 * ihh, F = 0.25 * ihh;
+    R.Lr[0] = A + B;  R.Ln[0] = -Cc - D;  R.Lt[0] = -E;  R.Le[0] = F;
+    R.Lr[1] = 0.5 - (0.5 * R.uu0) * ihh;  R.Ln[1] = (0.5 * R.un0) * ihh;  R.Lt[1] = (0.5 * R.ut0) * ihh;
+    R.Le[1] = -0.5 * ihh;
+    R.Lr[2] = A - B;  R.Ln[2] = -Cc + D;  R.Lt[2] = -E;  R.Le[2] = F;
 }
 
-// The 5-point split-flux stencil of one sign: + side F+(f-3+i), - side
-// F-(f+2-i), i = 0..4 (scheme_weno5z_general.f90:81, :106 with GLF fluxes).
+// The 5-point split-flux stencil of one sign, doubled (lf_flux2): + side
+// 2F+(f-3+i), - side 2F-(f+2-i), i = 0..4 (scheme_weno5z_general.f90:81, :106
+// with GLF fluxes).
 template <int DIR, int NC, int ST>
 __device__ __forceinline__ void flux_stencil(const double* __restrict__ s, int base, bool plus, double alpha,
                                              double F[5][4])
@@ -148,12 +153,13 @@ __device__ __forceinline__ void flux_stencil(const double* __restrict__ s, int b
         const double un = s[VN * NC + o];
         const double upa = un + sa;
         const double c4[4] = {s[o], s[NC + o], s[2 * NC + o], s[3 * NC + o]};
-        lf_flux<DIR>(upa, un, s[REC_P * NC + o], c4, F[i]);
+        lf_flux2<DIR>(upa, un, s[REC_P * NC + o], c4, F[i]);
     }
 }
 
-// Characteristic reconstruction of one sign: project the stencil on the left
-// eigenvectors, WENO-Z per characteristic field, back-project with R.
+// Characteristic reconstruction of one sign: project the doubled stencil on
+// the halved left eigenvectors (the reference's projections bit for bit),
+// WENO-Z per characteristic field, back-project with R.
 template <int DIR>
 __device__ __forceinline__ void char_recon(const double F[5][4], const RoeFace& R, double* r4)
 {
@@ -163,9 +169,9 @@ __device__ __forceinline__ void char_recon(const double F[5][4], const RoeFace& 
     for (int kc = 0; kc < 4; kc++) {
         double lf[5];
         if (kc == 2) {
-            // l(:,3) = (-u_t, [0|1], [1|0], 0): (-u_t)*F_rho + F_t
+            // l(:,3) = (-u_t, [0|1], [1|0], 0): ((-u_t)*2F_rho + 2F_t) * 0.5
 #pragma unroll
-            for (int i = 0; i < 5; i++) lf[i] = (-R.ut0) * F[i][0] + F[i][TAN];
+            for (int i = 0; i < 5; i++) lf[i] = ((-R.ut0) * F[i][0] + F[i][TAN]) * 0.5;
         } else {
             const int q = kc == 3 ? 2 : kc;
             double l4[4];
@@ -205,8 +211,9 @@ __device__ __forceinline__ void char_recon(const double F[5][4], const RoeFace& 
     r4[3] = acc_e;
 }
 
-// AdaWENO's shared-weight branch of one sign: every component reconstructed
-// with the indicator's WENO-Z weights (scheme_weno5th_SC.f90:111-124).
+// AdaWENO's shared-weight branch of one sign: every component of the doubled
+// stencil reconstructed with the indicator's WENO-Z weights, halved
+// (scheme_weno5th_SC.f90:111-124).
 __device__ __forceinline__ void cw_shared(const double F[5][4], double w0, double w1, double w2, double* r4)
 {
 #pragma unroll
@@ -228,6 +235,7 @@ __device__ __forceinline__ void cw_shared(const double F[5][4], double w0, doubl
 // (S' - P^2)^2 < (1 - delta)/delta P^4: no quotient, and one reciprocal for
 // the shared weights w_k = a'_k / S'.  theta <= 1 always, so delta >= 1 takes
 // the characteristic branch everywhere and delta <= 0 the shared-weight one.
+// The weights returned are halved (cw_shared applies them to a doubled stencil).
 template <int DIR, int NC, int ST>
 __device__ __forceinline__ bool adaweno_sign(const double* __restrict__ s, int base, bool plus, double alpha,
                                             double delta, double& w0, double& w1, double& w2)
@@ -261,7 +269,7 @@ __device__ __forceinline__ bool adaweno_sign(const double* __restrict__ s, int b
         shared = m * m < k * (p2 * p2);
     }
     if (shared) {
-        const double ys = rcp(sw);
+        const double ys = 0.5 * rcp(sw);  // halved: the weights multiply candidates of 2F
         w0 = aa0 * ys;
         w1 = aa1 * ys;
         w2 = aa2 * ys;
