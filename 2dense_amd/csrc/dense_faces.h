@@ -8,8 +8,8 @@
 namespace dense {
 
 // The characteristic faces (switch 104 and 105) reconstruct their two flux
-// signs one after the other at 3 waves per SIMD: 104 at 168 VGPRs with
-// 52 B/lane of spill, 73.6 us per stage launch on double Mach 1920x480,
+// signs one after the other at 3 waves per SIMD: 104 at 166 VGPRs with
+// 20 B/lane of spill, 72 us per stage launch on double Mach 1920x480,
 // against 90 us with the signs interleaved at 2 waves (243 VGPRs); AdaWENO
 // 200 vs 215 us on Rayleigh-Taylor 1024x4096 and 63 vs 69 us on double Mach
 // at t = 0.1, where serial signs also let each sign's indicator alone decide
