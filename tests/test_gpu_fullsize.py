@@ -27,6 +27,8 @@ pytestmark = pytest.mark.gpu
 
 
 def _threads():
+    # the cores this process may use, at most the 16 host cores a one-GPU lease
+    # is allotted (the GPU box can show more cores than the lease owns)
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
