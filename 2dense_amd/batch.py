@@ -124,10 +124,10 @@ class Batch:
 
     def run(self, nsteps: int, want_dts: bool = False):
         t = np.zeros(self.n)
-        hist = np.zeros((min(nsteps, 256), self.n)) if want_dts else None
+        hist = np.zeros((max(1, nsteps), self.n)) if want_dts else None
         self._check(self.lib.dense_batch_run(self.h, nsteps, _dp(hist) if want_dts else None, _dp(t)), self.h)
         self.t = t
-        return hist
+        return hist[:nsteps] if want_dts else None
 
     def set_schedule(self, tile: int, order: int = 0, grid_pct: int = 100):
         self._check(self.lib.dense_batch_set_schedule(self.h, tile, order, grid_pct), self.h)

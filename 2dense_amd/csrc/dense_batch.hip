@@ -721,6 +721,18 @@ extern "C" int dense_batch_run(dense_batch_handle b, int nsteps, double* dt_hist
         int rc = batch_prime(b);
         if (rc) return rc;
     }
+    if (dt_hist && nsteps > b->hist_cap) {
+        // the device history grows to the longest run a caller asks a history
+        // for; the captured graph carries its address and is captured again
+        BCHK(b, hipStreamSynchronize(b->stream));
+        double* p = nullptr;
+        if (hipMalloc(&p, sizeof(double) * (size_t)nsteps * b->n) != hipSuccess)
+            return bfail(b, DENSE_ERR_NOMEM, "dt history of %d steps: allocation failed", nsteps);
+        (void)hipFree(b->d_hist);
+        b->d_hist = p;
+        b->hist_cap = nsteps;
+        batch_release_graph(b);
+    }
     const int chunk = 16;
     if (b->graph_steps != chunk) {
         int rc = batch_build_graph(b, chunk);
@@ -750,10 +762,7 @@ extern "C" int dense_batch_run(dense_batch_handle b, int nsteps, double* dt_hist
         if (t_out) t_out[i] = h[i].t;
         if (h[i].error && err < 0) err = i;
     }
-    if (dt_hist) {
-        const int m = nsteps < b->hist_cap ? nsteps : b->hist_cap;
-        BCHK(b, hipMemcpy(dt_hist, b->d_hist, sizeof(double) * (size_t)m * b->n, hipMemcpyDeviceToHost));
-    }
+    if (dt_hist) BCHK(b, hipMemcpy(dt_hist, b->d_hist, sizeof(double) * (size_t)nsteps * b->n, hipMemcpyDeviceToHost));
     if (err >= 0) return bfail(b, DENSE_ERR_NONFINITE, "instance %d: non-finite wave speed", err);
     return DENSE_OK;
 }

@@ -229,8 +229,8 @@ int dense_batch_destroy(dense_batch_handle b);
 int dense_batch_set_state(dense_batch_handle b, int i, const double* cons, double t0);
 /* Download instance i's state (interior; ghost layers zero). */
 int dense_batch_get_state(dense_batch_handle b, int i, double* cons);
-/* nsteps steps of every instance, each with its own CFL dt.  dt_hist (nullable)
- * receives dt[step*n + i] for the first min(nsteps,256) steps; t_out (nullable)
+/* nsteps steps of every instance, each with its own CFL dt.  dt_hist (nullable,
+ * nsteps*n doubles) receives dt[step*n + i] for every step; t_out (nullable)
  * each instance's clock afterwards. */
 int dense_batch_run(dense_batch_handle b, int nsteps, double* dt_hist, double* t_out);
 /* Tile shape (0..3), instance order (0 as given, 1 largest first) and the

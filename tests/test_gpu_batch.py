@@ -46,6 +46,28 @@ def test_batch_equals_single_instances(splitting, recon):
             assert b.t[i] == pytest.approx(dts.sum(), rel=0, abs=0) or b.t[i] == np.cumsum(dts)[-1]
 
 
+def test_batch_dt_history_covers_every_step():
+    """dense_batch_run returns every step's dt per instance for any run length
+    (the device history grows past its initial 256 steps), equal to the same
+    instances run alone; a second, shorter run reuses it."""
+    B = _batch_mod()
+    D = dense()
+    cfgs, states, _ = _ensemble(n=2, seed=3)
+    with B.Batch(cfgs) as b:
+        for i, c in enumerate(states):
+            b.set_state(i, c)
+        hist = b.run(300, want_dts=True)
+        hist2 = b.run(40, want_dts=True)
+    assert hist.shape == (300, 2) and hist2.shape == (40, 2)
+    for i in range(2):
+        with D.Solver(cfgs[i]) as s:
+            s.set_state(states[i])
+            n, dts = s.run(340, want_dts=True)
+        assert n == 340
+        assert np.array_equal(hist[:, i], dts[:300]), f"instance {i}: first run"
+        assert np.array_equal(hist2[:, i], dts[300:]), f"instance {i}: second run"
+
+
 @pytest.mark.parametrize("tile", [0, 1, 2, 3])
 @pytest.mark.parametrize("order,grid", [(0, 100), (1, 50)])
 def test_batch_schedule_invariance(tile, order, grid):
