@@ -265,8 +265,10 @@ struct dense_batch {
     int tile = 0, order = 0, grid_pct = 100;
     int ntiles = 0;
     bool need_max = true;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph = nullptr;      // BATCH_CHUNK steps
     int graph_steps = 0;
+    hipGraphExec_t graph_rem[16] = {};   // [r]: r < BATCH_CHUNK steps, the remainder of a run
+    int graph_rem_steps[16] = {};
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     std::vector<BatchInstState> ist_host;
@@ -296,6 +298,11 @@ static void batch_release_graph(dense_batch* b)
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
     b->graph_steps = 0;
+    for (int r = 0; r < 16; r++) {
+        if (b->graph_rem[r]) (void)hipGraphExecDestroy(b->graph_rem[r]);
+        b->graph_rem[r] = nullptr;
+        b->graph_rem_steps[r] = 0;
+    }
 }
 
 // Tile list for the current (tile shape, instance order): instances in the
@@ -692,9 +699,15 @@ extern "C" int dense_batch_init_ic(dense_batch_handle b, const dense_ic* ics)
     return DENSE_OK;
 }
 
-static int batch_build_graph(dense_batch* b, int nsteps)
+constexpr int BATCH_CHUNK = 16;
+static_assert(BATCH_CHUNK <= 16, "one cached remainder graph per count below the chunk");
+
+// Capture nsteps steps into *exec (replacing what it held).
+static int batch_build_graph(dense_batch* b, int nsteps, hipGraphExec_t* exec, int* exec_steps)
 {
-    batch_release_graph(b);
+    if (*exec) (void)hipGraphExecDestroy(*exec);
+    *exec = nullptr;
+    *exec_steps = 0;
     hipGraph_t g = nullptr;
     BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < nsteps; i++) {
@@ -706,10 +719,13 @@ static int batch_build_graph(dense_batch* b, int nsteps)
         }
     }
     BCHK(b, hipStreamEndCapture(b->stream, &g));
-    hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    hipError_t e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return bfail(b, DENSE_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-    b->graph_steps = nsteps;
+    if (e != hipSuccess) {
+        *exec = nullptr;
+        return bfail(b, DENSE_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    *exec_steps = nsteps;
     return DENSE_OK;
 }
 
@@ -733,9 +749,14 @@ extern "C" int dense_batch_run(dense_batch_handle b, int nsteps, double* dt_hist
         b->hist_cap = nsteps;
         batch_release_graph(b);
     }
-    const int chunk = 16;
-    if (b->graph_steps != chunk) {
-        int rc = batch_build_graph(b, chunk);
+    // whole chunks, then one graph of the remainder: no replayed step is idle
+    const int full = nsteps / BATCH_CHUNK, rem = nsteps % BATCH_CHUNK;
+    if (full > 0 && b->graph_steps != BATCH_CHUNK) {
+        int rc = batch_build_graph(b, BATCH_CHUNK, &b->graph, &b->graph_steps);
+        if (rc) return rc;
+    }
+    if (rem > 0 && b->graph_rem_steps[rem] != rem) {
+        int rc = batch_build_graph(b, rem, &b->graph_rem[rem], &b->graph_rem_steps[rem]);
         if (rc) return rc;
     }
     // clear the per-instance step counters used to index the dt history
@@ -750,7 +771,8 @@ extern "C" int dense_batch_run(dense_batch_handle b, int nsteps, double* dt_hist
     hipLaunchKernelGGL(batch_set_run_kernel, dim3(1), dim3(1), 0, b->stream, b->d_bst, nsteps);
     BCHK(b, hipGetLastError());
     BCHK(b, hipEventRecord(b->ev0, b->stream));
-    for (int i = 0; i < (nsteps + chunk - 1) / chunk; i++) BCHK(b, hipGraphLaunch(b->graph, b->stream));
+    for (int i = 0; i < full; i++) BCHK(b, hipGraphLaunch(b->graph, b->stream));
+    if (rem > 0) BCHK(b, hipGraphLaunch(b->graph_rem[rem], b->stream));
     BCHK(b, hipEventRecord(b->ev1, b->stream));
     BCHK(b, hipStreamSynchronize(b->stream));
     float ms = 0.f;
