@@ -130,6 +130,29 @@ def test_rankine_hugoniot_post_shock_state():
         assert rho2 > 1.0 and p2 > 1.0 and u2 < u1
 
 
+def test_ensemble_vortex_reduces_to_the_reference_state(oracle):
+    """The ensemble's vortex builder at the reference's benchmark-1 parameters
+    (strength b = 0.5 at (5, 5) on [0, 10]^2, no free stream,
+    init_IVC_general.f90:12-43) is the reference's initial state, restated by
+    the oracle and pinned by the ivc_* fixtures, to rounding: the builder's
+    minimum-image distance is the plain distance on this box, and the two
+    differ only in the order of exp, ** and the density exponent (2.5 there,
+    1/(gamma - 1) here).  A dropped factor, a sign of the rotation or a wrong
+    exponent fails it."""
+    B = importlib.import_module("2dense_amd.batch")
+    for n in (32, 200):
+        o = oracle.Oracle(oracle.problem_params(1, n, n))
+        o.init_state(1)
+        ref = o.interior()
+        o.close()
+        got, h = B.vortex_state(n, 0.5, 5.0, 5.0, uinf=0.0, vinf=0.0)
+        assert h == 10.0 / n
+        scale = np.max(np.abs(ref[1]))  # |rho u| and |rho v| share it
+        for k in range(4):
+            s = scale if k in (1, 2) else np.max(np.abs(ref[k]))
+            assert np.max(np.abs(got[k] - ref[k])) <= 1e-14 * s, k
+
+
 def test_ensemble_shock_vortex_reduces_to_the_reference_state(oracle):
     """The ensemble's shock/vortex builder at the reference's parameters is the
     reference's benchmark-9 state (init_shockvortex_general.f90:34-51, restated
