@@ -224,7 +224,7 @@ def _mirror_defect_x(u):
 
 @pytest.mark.timeout(600)
 @pytest.mark.skipif(os.environ.get("DENSE_LONG") != "1",
-                    reason="Rayleigh-Taylor 1024x4096 to t = 1.95: 68,345 steps, about 85 s of GPU time, with no "
+                    reason="Rayleigh-Taylor 1024x4096 to t = 1.95: about 68,350 steps, about 40 s of GPU time, with no "
                            "reference run to compare (the end-time parity tests against the reference's own runs "
                            "are tests/test_gpu_endtime.py); set DENSE_LONG=1")
 def test_rt_1024x4096_to_configured_end_time():
