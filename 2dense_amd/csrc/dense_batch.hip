@@ -810,14 +810,26 @@ extern "C" int dense_batch_autotune(dense_batch_handle b, int trial_steps, int* 
     BCHK(b, hipStreamSynchronize(b->stream));
     long long total = 0;
     for (int i = 0; i < b->n; i++) total += 4 * b->inst[i].plane;
-    // save U^n of every instance and the clocks
+    // save U^n of every instance and the clocks (on failure: free what was
+    // taken, states and schedule untouched)
     std::vector<double*> save(b->n, nullptr);
-    for (int i = 0; i < b->n; i++) {
-        BCHK(b, hipMalloc(&save[i], sizeof(double) * 4 * b->inst[i].plane));
-        BCHK(b, hipMemcpy(save[i], b->inst[i].U[0], sizeof(double) * 4 * b->inst[i].plane, hipMemcpyDeviceToDevice));
-    }
     std::vector<BatchInstState> ist0(b->n);
-    BCHK(b, hipMemcpy(ist0.data(), b->d_ist, sizeof(BatchInstState) * b->n, hipMemcpyDeviceToHost));
+    const int t_in = b->tile, o_in = b->order, g_in = b->grid_pct;
+    int rc = DENSE_OK;
+    for (int i = 0; i < b->n && !rc; i++) {
+        if (hipMalloc(&save[i], sizeof(double) * 4 * b->inst[i].plane) != hipSuccess)
+            rc = bfail(b, DENSE_ERR_NOMEM, "autotune: state snapshot allocation failed");
+        else if (hipMemcpy(save[i], b->inst[i].U[0], sizeof(double) * 4 * b->inst[i].plane,
+                           hipMemcpyDeviceToDevice) != hipSuccess)
+            rc = bfail(b, DENSE_ERR_HIP, "autotune: state snapshot failed");
+    }
+    if (!rc && hipMemcpy(ist0.data(), b->d_ist, sizeof(BatchInstState) * b->n, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = bfail(b, DENSE_ERR_HIP, "autotune: clock snapshot failed");
+    if (rc) {
+        for (double* p : save)
+            if (p) (void)hipFree(p);
+        return rc;
+    }
     auto restore = [&]() -> int {
         for (int i = 0; i < b->n; i++)
             BCHK(b, hipMemcpy(b->inst[i].U[0], save[i], sizeof(double) * 4 * b->inst[i].plane, hipMemcpyDeviceToDevice));
@@ -830,7 +842,6 @@ extern "C" int dense_batch_autotune(dense_batch_handle b, int trial_steps, int* 
     const int tiles[] = {0, 1, 2, 3};
     const int orders[] = {0, 1};
     const int grids[] = {100, 50};
-    int rc = DENSE_OK;
     for (int t : tiles)
         for (int o : orders)
             for (int g : grids) {
@@ -846,6 +857,13 @@ extern "C" int dense_batch_autotune(dense_batch_handle b, int trial_steps, int* 
             }
     rc = dense_batch_set_schedule(b, bt, bo, bg);
 done:
+    if (rc) {
+        // a failed trial: the states and clocks as they were, the caller's schedule
+        const int keep = rc;
+        (void)restore();
+        (void)dense_batch_set_schedule(b, t_in, o_in, g_in);
+        rc = keep;
+    }
     for (double* p : save)
         if (p) (void)hipFree(p);
     if (rc) return rc;
